@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Headline benchmark: LaplCube Dirichlet solves/s at 128^3 (interior 127^3).
+
+Metric (the north star's): solves/s of FP32+refine solves reaching
+||f - A u||_2/||f||_2 <= 1e-10 (FP64), with the plain FP64 GPU solve timed
+beside it, plus the CPU reference timed on the host cores.
+
+A "step" is one batched solve of `--batch` independent 127^3 problems
+(fdmgpu_solve_batch_device), inputs already resident in HBM.  Inputs are the
+workload's seeded manufactured fields: u ~ U[-1,1) from a counter-based
+splitmix64 (seed s + problem index), f = the 7-point discrete Laplacian of u
+with zero Dirichlet boundary, geometry [-pi,pi]^3 with h = 2pi/127,
+l = 128 h.
+
+Prints ONE JSON line (see DESIGN.md for the byte model behind `roofline`).
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+# ---------------------------------------------------------------------------
+# workload generation (numpy; identical definition to the test fixtures)
+# ---------------------------------------------------------------------------
+def splitmix_field(seed: int, n: int) -> np.ndarray:
+    idx = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + (idx + np.uint64(1)) * np.uint64(0xD1B54A32D192ED03)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return 2.0 * (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53 - 1.0
+
+
+def lapl7(u: np.ndarray, h: float) -> np.ndarray:
+    p = np.pad(u, 1)
+    return (p[1:-1, 1:-1, :-2] + p[1:-1, 1:-1, 2:] + p[1:-1, :-2, 1:-1] + p[1:-1, 2:, 1:-1]
+            + p[:-2, 1:-1, 1:-1] + p[2:, 1:-1, 1:-1] - 6.0 * u) / (h * h)
+
+
+def manufactured(n: int, seed: int):
+    h = 2 * math.pi / n
+    u = splitmix_field(seed, n ** 3).reshape(n, n, n)
+    return u, lapl7(u, h), h, (n + 1) * h
+
+
+def unixbench_score(ts):
+    """geometric mean of the fastest 2/3 (src/unixbench_score.cpp:9-20)."""
+    ts = sorted(ts)
+    keep = max(1, (2 * len(ts)) // 3)
+    return math.exp(sum(math.log(t) for t in ts[:keep]) / keep)
+
+
+# ---------------------------------------------------------------------------
+# GPU legs
+# ---------------------------------------------------------------------------
+def algorithmic_bytes(kernel: str, mode: str, iters: int, n: int, batch: int) -> float:
+    """Algorithmic HBM bytes per solve of `batch` problems for one kernel class.
+
+    Per point: FP64 plain: xy_fwd 8+8, zcr 8+8, xy_inv 8+8.
+    FP32+refine, iteration 1: xy_fwd f64->f32 8+4, zcr 4+4, xy_inv f32->f64 4+8,
+    residual (read f, u; write r32) 8+8+4; iterations >= 2: xy_fwd 4+4, zcr 4+4,
+    xy_inv with update (read c, u; write u) 4+8+8, residual 8+8+4.
+    """
+    pts = float(n) ** 3 * batch
+    if mode == "f64":
+        per = {"xy_fwd": 16, "zcr": 16, "xy_inv": 16}.get(kernel, 0)
+        return per * pts
+    first = {"xy_fwd": 12, "zcr": 8, "xy_inv": 12, "residual": 20}.get(kernel, 0)
+    later = {"xy_fwd": 8, "zcr": 8, "xy_inv": 20, "residual": 20}.get(kernel, 0)
+    return (first + later * max(0, iters - 1)) * pts
+
+
+def run_gpu(mode: str, n: int, batch: int, steps: int, warmup: int, f_host: np.ndarray):
+    import fdm_amd
+    from fdm_amd import hip
+
+    h = 2 * math.pi / n
+    l = (n + 1) * h
+    solver = fdm_amd.LaplCube(h, h, h, l, l, l, n, n, n, precision="refine" if mode == "refine" else "f64")
+    d_f = hip.DeviceArray.from_host(f_host)
+    d_u = hip.DeviceArray(f_host.shape, np.float64)
+    stream = hip.Stream()
+    for _ in range(warmup):
+        solver.solve_batch(d_u, d_f, stream)
+    stream.synchronize()
+    solver.set_timing(True)
+    ev0, ev1 = hip.Event(), hip.Event()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        solver.solve_batch(d_u, d_f, stream)
+    ev1.record(stream)
+    stream.synchronize()
+    hip.synchronize()
+    t1 = time.perf_counter()
+    wall_ms = (t1 - t0) * 1e3
+    ev_ms = ev0.elapsed_ms(ev1)
+    timing = solver.timing()
+    st = solver.stats()
+    u = d_u.download()
+    iters = st.iterations if mode == "refine" else 1
+    return {
+        "wall_ms": wall_ms, "event_ms": ev_ms, "timing": timing, "iterations": iters,
+        "max_rel_residual": st.max_rel_residual, "u0": u[0], "u_last": u[-1],
+    }
+
+
+def cpu_baseline(n: int, f0: np.ndarray, budget_s: float = 15.0):
+    """The reference's own LaplCube<double,false> (oracle/_ref, built from the reference
+    sources) when present, else the oracle restatement; OpenMP over the host cores."""
+    import oracle as O
+    h = 2 * math.pi / n
+    l = (n + 1) * h
+    kind = "reference" if O.ref_available() else "port"
+    threads = O.lib().orc_num_threads()
+    times = []
+    ans = np.zeros_like(f0)
+    if kind == "reference":
+        R = O.ref()
+        hd = R.ref_cube_create_d(h, h, h, l, l, l, n, n, n)
+        rhs = f0.copy()
+        R.ref_cube_solve_d(hd, ans, rhs)   # warm-up
+        t_start = time.perf_counter()
+        while len(times) < 20 or (time.perf_counter() - t_start < budget_s and len(times) < 200):
+            rhs = f0.copy()
+            t = time.perf_counter()
+            R.ref_cube_solve_d(hd, ans, rhs)
+            times.append(time.perf_counter() - t)
+            if time.perf_counter() - t_start > 4 * budget_s:
+                break
+        R.ref_cube_destroy(hd)
+    else:
+        O.lapl_cube(f0, h, h, h, l, l, l)
+        t_start = time.perf_counter()
+        while len(times) < 20 or (time.perf_counter() - t_start < budget_s and len(times) < 200):
+            t = time.perf_counter()
+            ans = O.lapl_cube(f0, h, h, h, l, l, l)
+            times.append(time.perf_counter() - t)
+            if time.perf_counter() - t_start > 4 * budget_s:
+                break
+    score = unixbench_score(times)
+    med = statistics.median(times)
+    return {
+        "value": 1.0 / score, "unit": "solves/s", "cores": threads, "kind": kind,
+        "sample": f"{len(times)} LaplCube<double> Dirichlet {n}^3 solves after 1 warm-up, one at a time; "
+                  f"value = 1/unixbench_score (geomean of fastest 2/3); median {1.0 / med:.2f} solves/s",
+        "median_solves_per_s": 1.0 / med,
+    }, ans
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=32, help="127^3 problems per step")
+    ap.add_argument("--n", type=int, default=127)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the batched solve shards by whole "
+                                   "problems across GPUs (see DESIGN.md); run with --gpus 1"}))
+        sys.exit(2)
+    n, B = args.n, args.batch
+
+    # inputs: B seeded manufactured problems (seed 3 + i), as in the batched workload
+    us, fs = [], []
+    for i in range(B):
+        u, f, _, _ = manufactured(n, 3 + i)
+        fs.append(f)
+        if i in (0, B - 1):
+            us.append(u)
+    F = np.stack(fs)
+    del fs
+
+    res_r = run_gpu("refine", n, B, args.steps, args.warmup, F)
+    res_d = run_gpu("f64", n, B, args.steps, args.warmup, F)
+
+    solves = B * args.steps
+    value = solves / (res_r["wall_ms"] / 1e3)
+    fp64_value = solves / (res_d["wall_ms"] / 1e3)
+
+    # dominant kernel of the headline (refine) path and its roofline
+    tm = res_r["timing"]
+    dom = max((k for k in tm if k != "other"), key=lambda k: tm[k][0])
+    ms_total, launches = tm[dom]
+    alg = algorithmic_bytes(dom, "refine", res_r["iterations"], n, B) * args.steps
+    achieved = alg / (ms_total / 1e3) / 1e9
+    # whole-solve byte model 48 k n (k inner solves)
+    model_bytes = 48.0 * res_r["iterations"] * n ** 3 * solves
+    err_u = float(np.linalg.norm(res_d["u0"] - us[0]) / np.linalg.norm(us[0]))
+    err_r = float(np.linalg.norm(res_r["u0"] - res_d["u0"]) / np.linalg.norm(res_d["u0"]))
+
+    out = {
+        "metric": "solves/s @128^3 to 1e-10 rel (FP32+refine)",
+        "value": round(value, 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res_r["wall_ms"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 solve + fp64 residual/update",
+        "data": "synthetic (seeded splitmix64 manufactured fields)",
+        "config": {"workload": f"LaplCube Dirichlet {n}^3 interior (N={n + 1}) FP32+refine to 1e-10, "
+                               f"batch of {B} per step", "n": n, "batch": B, "tolerance": 1e-10},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(ms_total / launches, 5), "launches": launches},
+        "iterations": res_r["iterations"],
+        "max_rel_residual": res_r["max_rel_residual"],
+        "fp64_solves_per_s": round(fp64_value, 2),
+        "refine_over_fp64": round(value / fp64_value, 3),
+        "err_fp64_vs_manufactured": err_u,
+        "err_refine_vs_fp64_gpu": err_r,
+        "solve_model_GBs": round(model_bytes / (res_r["wall_ms"] / 1e3) / 1e9, 1),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tm.items()},
+        "fp64_kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in res_d["timing"].items()},
+    }
+    if not args.no_cpu:
+        cb, ans_cpu = cpu_baseline(n, F[0], args.cpu_budget)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["err_fp64_gpu_vs_cpu"] = float(np.linalg.norm(res_d["u0"] - ans_cpu) / np.linalg.norm(ans_cpu))
+        out["err_refine_vs_cpu"] = float(np.linalg.norm(res_r["u0"] - ans_cpu) / np.linalg.norm(ans_cpu))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

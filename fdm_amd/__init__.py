@@ -1,0 +1,284 @@
+"""fdm_amd: MI355X (gfx950) fast Poisson solve behind fdm's LaplCube / LaplRect API.
+
+Python mirror of the reference's solver classes (reference src/lapl_cube.h:9-106,
+src/lapl_rect.h:11-69) over the C ABI in include/fdmgpu.h, implemented by the
+HIP library fdm_amd/libfdmgpu.so.  There is no CPU fallback: if the library is
+missing or no GPU is present, constructing a solver raises.
+
+    from fdm_amd import LaplCube
+    s = LaplCube(dx, dy, dz, lx, ly, lz, nx, ny, nz, precision="f64")
+    s.solve(ans, rhs)     # numpy arrays (synchronous) or torch CUDA tensors (stream-ordered)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libfdmgpu.so")
+
+F32, F64, F32_REFINE64 = 0, 1, 2
+PERIODIC_X, PERIODIC_Y, PERIODIC_Z = 1, 2, 4
+_PREC = {"f32": F32, "float": F32, "f64": F64, "double": F64, "refine": F32_REFINE64,
+         "f32_refine64": F32_REFINE64}
+
+# error codes (include/fdmgpu.h)
+OK, ERR_INVALID_SIZE, ERR_INVALID_ARG, ERR_UNSUPPORTED = 0, 1, 2, 3
+ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_ADMISSION = 4, 5, 6, 7, 8
+
+# every entry point include/fdmgpu.h declares
+EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_rect_set_scales", "fdmgpu_set_refine",
+           "fdmgpu_solve_host", "fdmgpu_solve_device", "fdmgpu_solve_batch_device", "fdmgpu_get_stats",
+           "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_destroy",
+           "fdmgpu_strerror", "fdmgpu_version")
+
+# kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
+K_XY_FWD, K_XY_INV, K_X_TILE, K_Y_TILE, K_ZCR, K_ROWCR, K_RESIDUAL, K_OTHER = range(8)
+KERNEL_NAMES = ("xy_fwd", "xy_inv", "x_tile", "y_tile", "zcr", "rowcr", "residual", "other")
+
+
+class FdmGpuError(RuntimeError):
+    """A non-zero fdmgpu return code (the reference aborts via verify(), src/verify.h)."""
+
+    def __init__(self, code: int, where: str):
+        self.code = code
+        super().__init__(f"{where}: {_lib_handle().fdmgpu_strerror(code).decode()} (code {code})")
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("rel_residual", C.c_double), ("max_rel_residual", C.c_double)]
+
+
+def build(force: bool = True) -> None:
+    """Compile libfdmgpu.so for gfx950 with hipcc (make -C fdm_amd)."""
+    args = ["make", "-s", "-j8", "-C", _HERE]
+    if force:
+        args.append("-B")
+    subprocess.check_call(args)
+
+
+_lib = None
+
+
+def _lib_handle():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is not built: run `make -C fdm_amd` (HIP library required, "
+                              "there is no CPU fallback)")
+        L = C.CDLL(LIBPATH)
+        vp, d, i = C.c_void_p, C.c_double, C.c_int
+        L.fdmgpu_plan_cube.argtypes = [C.POINTER(vp), i, i, d, d, d, d, d, d, i, i, i]
+        L.fdmgpu_plan_rect.argtypes = [C.POINTER(vp), i, i, d, d, d, d, i, i]
+        L.fdmgpu_rect_set_scales.argtypes = [vp, vp, vp, vp]
+        L.fdmgpu_set_refine.argtypes = [vp, d, i]
+        L.fdmgpu_solve_host.argtypes = [vp, vp, vp]
+        L.fdmgpu_solve_device.argtypes = [vp, vp, vp, vp]
+        L.fdmgpu_solve_batch_device.argtypes = [vp, i, vp, vp, vp]
+        L.fdmgpu_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.fdmgpu_plan_footprint.argtypes = [vp, i, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
+        L.fdmgpu_set_timing.argtypes = [vp, i]
+        L.fdmgpu_get_timing.argtypes = [vp, i, C.POINTER(d), C.POINTER(i)]
+        L.fdmgpu_destroy.argtypes = [vp]
+        L.fdmgpu_strerror.argtypes = [i]
+        L.fdmgpu_strerror.restype = C.c_char_p
+        L.fdmgpu_version.restype = i
+        _lib = L
+    return _lib
+
+
+def lib():
+    """The loaded libfdmgpu.so (ctypes)."""
+    return _lib_handle()
+
+
+def _check(rc: int, where: str):
+    if rc != OK:
+        raise FdmGpuError(rc, where)
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _is_dev(a) -> bool:
+    return type(a).__name__ == "DeviceArray"
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return C.c_void_p()
+    return getattr(stream, "h", stream)
+
+
+class _Plan:
+    _kind = ""
+
+    def __init__(self, handle: C.c_void_p, precision: int, shape):
+        self._h = handle
+        self.precision = precision
+        self.shape = tuple(shape)
+        self.dtype = np.float32 if precision == F32 else np.float64
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib_handle().fdmgpu_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- solve --------------------------------------------------------------
+    def _check_array(self, a, name):
+        if tuple(a.shape) != self.shape:
+            raise ValueError(f"{name} has shape {tuple(a.shape)}, expected {self.shape}")
+
+    def solve(self, ans, rhs, stream=None):
+        """Reference semantics: ans = A^{-1} rhs; ans fully overwritten; ans may be rhs.
+
+        numpy arrays: synchronous host solve (fdmgpu_solve_host).
+        fdm_amd.hip.DeviceArray: enqueued on `stream` (fdmgpu_solve_device).
+        torch CUDA tensors: enqueued on torch's current stream.  torch ships its
+        own HIP runtime: import torch and initialise its device before this
+        package loads libfdmgpu.so so that both share one runtime.
+        """
+        self._check_array(ans, "ans")
+        self._check_array(rhs, "rhs")
+        L = _lib_handle()
+        if _is_dev(ans):
+            for a, nm in ((ans, "ans"), (rhs, "rhs")):
+                if a.dtype != self.dtype:
+                    raise TypeError(f"{nm} must be {np.dtype(self.dtype).name}")
+            _check(L.fdmgpu_solve_device(self._h, ans.ptr, rhs.ptr, _stream_handle(stream)), f"{self._kind}.solve")
+            return ans
+        if _is_torch(ans):
+            import torch
+            want = torch.float32 if self.precision == F32 else torch.float64
+            for t, nm in ((ans, "ans"), (rhs, "rhs")):
+                if t.dtype != want or not t.is_cuda or not t.is_contiguous():
+                    raise TypeError(f"{nm} must be a contiguous CUDA {want} tensor")
+            stream = torch.cuda.current_stream(ans.device).cuda_stream
+            _check(L.fdmgpu_solve_device(self._h, C.c_void_p(ans.data_ptr()), C.c_void_p(rhs.data_ptr()),
+                                         C.c_void_p(stream)), f"{self._kind}.solve")
+            return ans
+        for a, nm in ((ans, "ans"), (rhs, "rhs")):
+            if a.dtype != self.dtype or not a.flags["C_CONTIGUOUS"]:
+                raise TypeError(f"{nm} must be a C-contiguous {np.dtype(self.dtype).name} array")
+        if not ans.flags["WRITEABLE"]:
+            raise TypeError("ans must be writeable")
+        _check(L.fdmgpu_solve_host(self._h, ans.ctypes.data_as(C.c_void_p), rhs.ctypes.data_as(C.c_void_p)),
+               f"{self._kind}.solve")
+        return ans
+
+    def solve_batch(self, ans, rhs, stream=None):
+        """count problems stacked on a leading axis (DeviceArray or torch CUDA tensors),
+        solved in one launch sequence."""
+        count = ans.shape[0]
+        if tuple(ans.shape[1:]) != self.shape or tuple(rhs.shape) != tuple(ans.shape):
+            raise ValueError("batch shape mismatch")
+        if _is_dev(ans):
+            pa, pr, st = ans.ptr, rhs.ptr, _stream_handle(stream)
+        else:
+            import torch
+            pa, pr = C.c_void_p(ans.data_ptr()), C.c_void_p(rhs.data_ptr())
+            st = C.c_void_p(torch.cuda.current_stream(ans.device).cuda_stream)
+        _check(_lib_handle().fdmgpu_solve_batch_device(self._h, count, pa, pr, st), f"{self._kind}.solve_batch")
+        return ans
+
+    def set_timing(self, enable: bool = True):
+        """Enable (and reset) per-kernel HIP-event timing."""
+        _check(_lib_handle().fdmgpu_set_timing(self._h, int(enable)), "set_timing")
+
+    def timing(self):
+        """{kernel name: (total ms, launches)} since the last set_timing (synchronises)."""
+        out = {}
+        for k, name in enumerate(KERNEL_NAMES):
+            ms, n = C.c_double(), C.c_int()
+            _check(_lib_handle().fdmgpu_get_timing(self._h, k, C.byref(ms), C.byref(n)), "get_timing")
+            if n.value:
+                out[name] = (ms.value, n.value)
+        return out
+
+    def set_refine(self, rel_tol: float = 1e-10, max_iter: int = 4):
+        _check(_lib_handle().fdmgpu_set_refine(self._h, rel_tol, max_iter), "set_refine")
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(_lib_handle().fdmgpu_get_stats(self._h, C.byref(st)), "get_stats")
+        return st
+
+    def footprint(self, count: int = 1):
+        r, p = C.c_ulonglong(), C.c_ulonglong()
+        _check(_lib_handle().fdmgpu_plan_footprint(self._h, count, C.byref(r), C.byref(p)), "footprint")
+        return r.value, p.value
+
+
+def _prec(precision) -> int:
+    if isinstance(precision, int):
+        return precision
+    if precision in (np.float32, np.dtype(np.float32)):
+        return F32
+    if precision in (np.float64, np.dtype(np.float64)):
+        return F64
+    return _PREC[str(precision).lower()]
+
+
+class LaplCube(_Plan):
+    """LaplCube<T,check,F>(dx,dy,dz,lx,ly,lz,nx,ny,nz) (reference src/lapl_cube.h:58-100).
+
+    Arrays are [nz][ny][nx] interior points, x fastest.  precision: "f32"
+    (LaplCube<float>), "f64" (LaplCube<double>) or "refine" (double in/out,
+    FP32 solves with FP64 iterative refinement).  Transform lengths nx+1,
+    ny+1, nz+1 must be powers of two (the reference aborts otherwise,
+    src/fft.cpp:67).
+    """
+    _kind = "LaplCube"
+
+    def __init__(self, dx, dy, dz, lx, ly, lz, nx, ny, nz, precision="f64", periodic=False):
+        h = C.c_void_p()
+        p = _prec(precision)
+        _check(_lib_handle().fdmgpu_plan_cube(C.byref(h), p, 7 if periodic else 0, dx, dy, dz, lx, ly, lz,
+                                              nx, ny, nz), "LaplCube")
+        super().__init__(h, p, (nz, ny, nx))
+        self.dx, self.dy, self.dz, self.lx, self.ly, self.lz = dx, dy, dz, lx, ly, lz
+        self.nx, self.ny, self.nz = nx, ny, nz
+
+
+class LaplRect(_Plan):
+    """LaplRect<T,check,F>(dx,dy,lx,ly,nx,ny) (reference src/lapl_rect.h:40-63).
+
+    Arrays are [ny][nx], x fastest.  lm_y_scale / L_scale / U_scale (length
+    nx+1, index 0 unused, default ones) via set_scales (src/lapl_rect.h:57-59).
+    """
+    _kind = "LaplRect"
+
+    def __init__(self, dx, dy, lx, ly, nx, ny, precision="f64", yperiodic=False):
+        h = C.c_void_p()
+        p = _prec(precision)
+        _check(_lib_handle().fdmgpu_plan_rect(C.byref(h), p, PERIODIC_Y if yperiodic else 0, dx, dy, lx, ly, nx,
+                                              ny), "LaplRect")
+        super().__init__(h, p, (ny, nx))
+        self.dx, self.dy, self.lx, self.ly, self.nx, self.ny = dx, dy, lx, ly, nx, ny
+
+    def set_scales(self, lm_y_scale=None, L_scale=None, U_scale=None):
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64)
+            if a.shape != (self.nx + 1,):
+                raise ValueError("scales have nx+1 entries (index 0 unused)")
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        _check(_lib_handle().fdmgpu_rect_set_scales(self._h, ptr(lm_y_scale), ptr(L_scale), ptr(U_scale)),
+               "LaplRect.set_scales")
+
+
+def version() -> int:
+    return _lib_handle().fdmgpu_version()

@@ -1,0 +1,350 @@
+// Device building blocks for the fast Poisson solve on gfx950 (CDNA4, wave64).
+//
+//  * dft<R>:        forward DFT of R <= 32 complex values held in registers
+//                   (fully unrolled radix-2, compile-time twiddles).
+//  * dst_pairs<>:   DST-I of pencils held in LDS, two pencils per job.
+//
+// DST-I definition (the transform FFT<T>::sFFT computes, reference
+// src/fft.cpp:293-365):  X[k] = scale * sum_{j=1}^{N-1} x[j] sin(pi j k / N),
+// k = 1..N-1, N = 2^LOGN.  Algorithm (not the reference's Samarskii-Nikolaev
+// recursion, which is sequential per pencil):
+//   1. w_j = sin(pi j/N)(x_j + x_{N-j}) + (x_j - x_{N-j})/2        (w_0 = 0)
+//      Two pencils a, b are packed as z_j = w^a_j + i w^b_j.
+//   2. Z = FFT_N(z), done as a 4-step FFT N = N1 x N2: N1 lanes per job each
+//      run an N2-point DFT in registers, twiddle by w_N^{j1 k2}, exchange
+//      through the pencils' own LDS slots, then N1-point DFTs.
+//   3. With W = the real DFT of w:  Re W_k = X_{2k+1} - X_{2k-1},
+//      Im W_k = -X_{2k}.  So X_{2k} comes directly and the odd outputs are a
+//      prefix sum X_{2k+1} = Re W_0 / 2 + sum_{i=1..k} Re W_i, computed as a
+//      per-lane running sum plus a shuffle scan across the job's N1 lanes.
+// Everything happens in place in the pencils' N LDS slots (slot 0 is a
+// dummy that reads as zero after the transform), so an N x N plane (one
+// dummy row, one dummy column) holds a full 127^2 FP64 plane in 139 KB.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+namespace fdmgpu {
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+constexpr int bitrev(int v, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; b++)
+        if (v & (1 << b)) r |= 1 << (bits - 1 - b);
+    return r;
+}
+
+// cos(2 pi m / 32) from the first quadrant by symmetry (exact zeros).
+constexpr double kQuarter[9] = {1.0,
+                                0.98078528040323044912618223613424,
+                                0.92387953251128675612818318939679,
+                                0.83146961230254523707878837761791,
+                                0.70710678118654752440084436210485,
+                                0.55557023301960222474283081394853,
+                                0.38268343236508977172845998403040,
+                                0.19509032201612826784828486847702,
+                                0.0};
+constexpr double cos32(int m) {
+    m = ((m % 32) + 32) % 32;
+    return m <= 8 ? kQuarter[m] : m <= 16 ? -kQuarter[16 - m] : m <= 24 ? -kQuarter[m - 16] : kQuarter[32 - m];
+}
+constexpr double sin32(int m) { return cos32(m - 8); }
+
+// (re, im) *= exp(-2 pi i K / L), L | 32, all compile time.
+template <int L, int K, typename T>
+__device__ __forceinline__ void twiddle_ct(T& re, T& im) {
+    constexpr int k = ((K % L) + L) % L;
+    if constexpr (k == 0) {
+    } else if constexpr (4 * k == L) {   // * (-i)
+        T t = re;
+        re = im;
+        im = -t;
+    } else if constexpr (2 * k == L) {
+        re = -re;
+        im = -im;
+    } else if constexpr (4 * k == 3 * L) {   // * (+i)
+        T t = re;
+        re = -im;
+        im = t;
+    } else {
+        constexpr int m = k * (32 / L);
+        constexpr T c = (T)cos32(m);
+        constexpr T s = (T)(-sin32(m));
+        T r = re * c - im * s;
+        T i = re * s + im * c;
+        re = r;
+        im = i;
+    }
+}
+
+// In-register forward DFT of size R (natural order in and out).
+template <int R, typename T>
+__device__ __forceinline__ void dft(T (&re)[R], T (&im)[R]) {
+    static_assert(R >= 1 && R <= 32 && (R & (R - 1)) == 0, "radix");
+    if constexpr (R > 1) {
+        constexpr int LOG = ilog2(R);
+        static_for<0, R>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            constexpr int r = bitrev(i, LOG);
+            if constexpr (i < r) {
+                T t = re[i];
+                re[i] = re[r];
+                re[r] = t;
+                t = im[i];
+                im[i] = im[r];
+                im[r] = t;
+            }
+        });
+        static_for<1, LOG + 1>([&](auto ss) {
+            constexpr int len = 1 << decltype(ss)::value;
+            static_for<0, R / len>([&](auto bb) {
+                static_for<0, len / 2>([&](auto kk) {
+                    constexpr int a = decltype(bb)::value * len + decltype(kk)::value;
+                    constexpr int b = a + len / 2;
+                    T tr = re[b], ti = im[b];
+                    twiddle_ct<len, decltype(kk)::value>(tr, ti);
+                    re[b] = re[a] - tr;
+                    im[b] = im[a] - ti;
+                    re[a] = re[a] + tr;
+                    im[a] = im[a] + ti;
+                });
+            });
+        });
+    }
+}
+
+template <int LOGN>
+struct DstShape {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int N1 = 1 << (LOGN / 2);   // lanes per job
+    static constexpr int N2 = N / N1;            // N2 >= N1
+    static constexpr int M = N / 2;
+    static constexpr int KC = M / N1;            // post-processing outputs per lane (>= 1)
+    // LDS padding: a pencil-to-pencil stride congruent to N1 mod 32 banks spreads
+    // the N1 lanes of neighbouring jobs over distinct banks.
+    static constexpr int PAD = N1;
+};
+
+// Per-axis constant tables in LDS: twr[m], twi[m] = cos, -sin(2 pi m / N),
+// sn[j] = sin(pi j / N), m, j in [0, N).
+template <typename T>
+struct DstTables {
+    const T* twr;
+    const T* twi;
+    const T* sn;
+};
+
+// Inclusive scan across `width` consecutive lanes (width a power of two <= 64).
+template <int WIDTH, typename T>
+__device__ __forceinline__ T lane_scan_incl(T v) {
+#pragma unroll
+    for (int d = 1; d < WIDTH; d <<= 1) {
+        T o = __shfl_up(v, d, WIDTH);
+        if ((threadIdx.x & (WIDTH - 1)) >= d) v += o;
+    }
+    return v;
+}
+
+// DST-I of 2*JOBS pencils in LDS: pencil p, slot j at lds[p*SP + j*SJ];
+// job g transforms pencils g and g + JOBS.  Slots 1..N-1 hold x_1..x_{N-1}
+// on entry and scale*X_1..scale*X_{N-1} on exit; slot 0 is ignored on entry
+// and 0 on exit.  Every thread of the block must call this (block barriers).
+template <typename T, int LOGN, int JOBS>
+__device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const int SP, const int SJ,
+                                          const DstTables<T> tb, const T scale, const int tid,
+                                          const int nthreads) {
+    using S = DstShape<LOGN>;
+    constexpr int N = S::N, N1 = S::N1, N2 = S::N2, KC = S::KC;
+    constexpr int WORK = JOBS * N1;
+    for (int base = 0; base < WORK; base += nthreads) {
+        const int w = base + tid;
+        const bool active = w < WORK;
+        const int g = active ? w / N1 : 0;
+        const int t = w & (N1 - 1);
+        T* pa = lds + g * SP;
+        T* pb = lds + (g + JOBS) * SP;
+
+        // ---- step 1: pre-processing + N2-point DFTs over j2 (j = t + N1*j2)
+        T yr[N2], yi[N2];
+        if (active) {
+#pragma unroll
+            for (int j2 = 0; j2 < N2; ++j2) {
+                const int j = t + N1 * j2;
+                if (j == 0) {
+                    yr[j2] = T(0);
+                    yi[j2] = T(0);
+                } else {
+                    const T a0 = pa[j * SJ], a1 = pa[(N - j) * SJ];
+                    const T b0 = pb[j * SJ], b1 = pb[(N - j) * SJ];
+                    const T s = tb.sn[j];
+                    yr[j2] = s * (a0 + a1) + T(0.5) * (a0 - a1);
+                    yi[j2] = s * (b0 + b1) + T(0.5) * (b0 - b1);
+                }
+            }
+            dft<N2>(yr, yi);
+#pragma unroll
+            for (int k2 = 1; k2 < N2; ++k2) {
+                const int m = t * k2;   // < N
+                const T c = tb.twr[m], s = tb.twi[m];
+                const T r = yr[k2] * c - yi[k2] * s;
+                const T i = yr[k2] * s + yi[k2] * c;
+                yr[k2] = r;
+                yi[k2] = i;
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int pos = t + N1 * k2;
+                pa[pos * SJ] = yr[k2];
+                pb[pos * SJ] = yi[k2];
+            }
+        }
+        __syncthreads();
+
+        // ---- step 2: N1-point DFTs over j1 for k2 = t + N1*r
+        constexpr int RR = N2 / N1;
+        T zr[N2], zi[N2];
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int k2 = t + N1 * r;
+                T vr[N1], vi[N1];
+#pragma unroll
+                for (int j1 = 0; j1 < N1; ++j1) {
+                    const int pos = j1 + N1 * k2;
+                    vr[j1] = pa[pos * SJ];
+                    vi[j1] = pb[pos * SJ];
+                }
+                dft<N1>(vr, vi);
+#pragma unroll
+                for (int k1 = 0; k1 < N1; ++k1) {
+                    zr[r * N1 + k1] = vr[k1];
+                    zi[r * N1 + k1] = vi[k1];
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int k2 = t + N1 * r;
+#pragma unroll
+                for (int k1 = 0; k1 < N1; ++k1) {
+                    const int k = k2 + N2 * k1;
+                    pa[k * SJ] = zr[r * N1 + k1];
+                    pb[k * SJ] = zi[r * N1 + k1];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- step 3: even outputs directly, odd outputs by prefix sum
+        T ae[KC], ao[KC], be[KC], bo[KC];
+        if (active) {
+            T runA = T(0), runB = T(0);
+#pragma unroll
+            for (int i = 0; i < KC; ++i) {
+                const int k = t * KC + i;
+                const int kn = (N - k) & (N - 1);
+                const T zkr = pa[k * SJ], zki = pb[k * SJ];
+                const T znr = pa[kn * SJ], zni = pb[kn * SJ];
+                ae[i] = T(0.5) * (zni - zki);
+                be[i] = T(0.5) * (zkr - znr);
+                T dA = T(0.5) * (zkr + znr);
+                T dB = T(0.5) * (zki + zni);
+                if (k == 0) {
+                    dA *= T(0.5);
+                    dB *= T(0.5);
+                }
+                runA += dA;
+                runB += dB;
+                ao[i] = runA;
+                bo[i] = runB;
+            }
+            if constexpr (N1 > 1) {
+                const T inclA = lane_scan_incl<N1>(runA);
+                const T inclB = lane_scan_incl<N1>(runB);
+                const T offA = inclA - runA, offB = inclB - runB;
+#pragma unroll
+                for (int i = 0; i < KC; ++i) {
+                    ao[i] += offA;
+                    bo[i] += offB;
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < KC; ++i) {
+                const int k = t * KC + i;
+                if (k == 0) {
+                    pa[0] = T(0);
+                    pb[0] = T(0);
+                } else {
+                    pa[(2 * k) * SJ] = ae[i] * scale;
+                    pb[(2 * k) * SJ] = be[i] * scale;
+                }
+                pa[(2 * k + 1) * SJ] = ao[i] * scale;
+                pb[(2 * k + 1) * SJ] = bo[i] * scale;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Constant-coefficient cyclic reduction along z for one (kx,ky) mode.
+//
+// After the DST in x and y, mode (kx,ky) obeys the Toeplitz tridiagonal system
+//   b u_{i-1} + d u_i + b u_{i+1} = g_i,  i = 0..n-1,  n = 2^Q - 1,
+//   b = 1/hz^2, d = -2/hz^2 - lambda_x[kx] - lambda_y[ky], u_{-1} = u_n = 0,
+// the system the reference's z DST + divide + inverse z DST solves
+// (src/lapl_cube.cpp:13-30, 70-80, 124-141).  Odd-even cyclic reduction
+// (Hockney & Jesshope, the reference's cyclic_reduction,
+// src/cyclic_reduction.h:20-56) keeps the coefficients uniform at every level
+// when n = 2^Q - 1, so each level is two scalars per mode:
+//   alpha_l = -c_{l-1}/d_{l-1},  d_l = d_{l-1} - 2 c_{l-1}^2/d_{l-1},
+//   c_l = -c_{l-1}^2/d_{l-1}.
+// G lanes share one system: lane c owns the contiguous chunk of rows
+// [c*C, (c+1)*C - 1) plus the separator row (c+1)*C - 1 (C = 2^Q/G).  The
+// lowest Q-g levels stay inside a chunk; the separators' updates are additive
+// so each lane accumulates its contribution to its two neighbouring
+// separators and one shuffle combines them; the remaining g-1 levels over the
+// G-1 separators are solved redundantly in every lane from shuffled values.
+// ---------------------------------------------------------------------------
+template <typename T, int Q>
+struct CrLevels {
+    T alpha[Q > 1 ? Q : 2];   // alpha[l], l = 1..Q-1
+    T c[Q];                   // c[l],  l = 0..Q-1
+    T inv_d[Q];               // 1/d[l], l = 0..Q-1
+};
+
+template <typename T, int Q>
+__device__ __forceinline__ void cr_levels(CrLevels<T, Q>& L, T b, T d) {
+    T cl = b, dl = d;
+    L.c[0] = cl;
+    L.inv_d[0] = T(1) / dl;
+#pragma unroll
+    for (int l = 1; l < Q; ++l) {
+        const T t = cl * cl * L.inv_d[l - 1];
+        L.alpha[l] = -cl * L.inv_d[l - 1];
+        dl = dl - T(2) * t;
+        cl = -t;
+        L.c[l] = cl;
+        L.inv_d[l] = T(1) / dl;
+    }
+}
+
+}  // namespace fdmgpu

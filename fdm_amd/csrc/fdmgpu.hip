@@ -1,0 +1,694 @@
+// C ABI of the MI355X fast Poisson solve (include/fdmgpu.h): plans, solves,
+// FP32+FP64-refinement driver.
+//
+// Solve pipeline for a Dirichlet cube (reference LaplCube::solve,
+// src/lapl_cube.cpp:9-142):
+//   1. DST-I along x and y of every z-plane, factors dx*slx, dy*sly
+//      (reference passes 3 and 2)                 -> plane kernel, or x + y tiles
+//   2. per (kx,ky) mode, the z tridiagonal system  -> zcr kernel
+//      (replaces reference passes 1, 4 and 7: z DST, divide by -sum(lambda),
+//      inverse z DST; same linear system, see dst_device.hpp)
+//   3. inverse DST-I along x and y, factors slx, sly (reference passes 5, 6)
+// The solve runs in place in `ans` (no scratch) for plain F32/F64 plans.
+// LaplRect (src/lapl_rect.cpp:63-110): y DST -> per-row cyclic reduction of
+// the gtsv system -> inverse y DST.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/fdmgpu.h"
+#include "kernels.hpp"
+#include "refine.hpp"
+
+using namespace fdmgpu;
+
+struct fdmgpu_plan_t {
+    int kind = 3;          // 3 = LaplCube, 2 = LaplRect
+    int precision = FDMGPU_F64;
+    int periodic = 0;
+    int nx = 0, ny = 0, nz = 1;
+    int logx = 0, logy = 0, logz = 0;   // log2 of the transform lengths
+    double dx = 1, dy = 1, dz = 1, lx = 1, ly = 1, lz = 1;
+    double hx_op = 1, hz_op = 1;        // spacings of the inverted operator (eigenvalue aliasing)
+    int device = 0;
+    size_t elem = 8;                    // bytes per element at the API
+    size_t csize = 8;                   // bytes per compute element
+    void* tab_x = nullptr;              // DST tables (compute type), x axis
+    void* tab_y = nullptr;              // y axis (== tab_x when the lengths match)
+    void* lmx = nullptr;                // eigenvalues per mode (compute type)
+    void* lmy = nullptr;
+    // LaplRect scales, per column (index 0..nx-1 = reference index 1..nx)
+    std::vector<double> h_sc, h_Ls, h_Us;
+    void* d_sc = nullptr;               // compute type
+    void* d_Ls = nullptr;
+    void* d_Us = nullptr;
+    double* d_sc64 = nullptr;           // FP64 copies for the residual
+    double* d_Ls64 = nullptr;
+    double* d_Us64 = nullptr;
+    bool scaled = false;
+    // refinement
+    double rtol = 1e-10;
+    int max_iter = 4;
+    float* r32 = nullptr;               // [cap][elems]
+    double* partial = nullptr;          // [cap][nblk][2]
+    int* done = nullptr;                // [cap]
+    int* iters = nullptr;
+    double* resid = nullptr;
+    int cap = 0;
+    double* fcopy = nullptr;            // f kept when a refine solve runs in place
+    size_t fcopy_bytes = 0;
+    // host staging
+    void* stage = nullptr;
+    size_t stage_bytes = 0;
+    hipStream_t own = nullptr;
+    hipStream_t last_stream = nullptr;
+    int last_count = 0;
+    // per-kernel timing (fdmgpu_set_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> evs;        // 2 per recorded launch
+    std::vector<int> rec_kernel;
+    size_t nrec = 0;
+    double tot_ms[FDMGPU_NUM_KERNELS] = {};
+    int launches[FDMGPU_NUM_KERNELS] = {};
+};
+
+namespace {
+
+thread_local char g_errbuf[256];
+
+int ilog2_exact(int N) {
+    if (N < 1 || (N & (N - 1))) return -1;
+    int l = 0;
+    while ((1 << l) < N) l++;
+    return l;
+}
+
+int hip_fail(hipError_t e) {
+    std::snprintf(g_errbuf, sizeof g_errbuf, "HIP error %d: %s", (int)e, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? FDMGPU_ERR_OUT_OF_MEMORY : FDMGPU_ERR_DEVICE;
+}
+
+#define HIPCHK(x)                              \
+    do {                                       \
+        hipError_t _e = (x);                   \
+        if (_e != hipSuccess) return hip_fail(_e); \
+    } while (0)
+
+template <typename T>
+int upload(void** dst, const std::vector<double>& v) {
+    std::vector<T> h(v.begin(), v.end());
+    if (!*dst) HIPCHK(hipMalloc(dst, std::max<size_t>(1, h.size()) * sizeof(T)));
+    HIPCHK(hipMemcpy(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FDMGPU_OK;
+}
+
+// twr[m] = cos(2 pi m/N), twi[m] = -sin(2 pi m/N), sn[j] = sin(pi j/N); m, j < N
+std::vector<double> dst_tables(int N) {
+    std::vector<double> t(3 * (size_t)N);
+    for (int m = 0; m < N; m++) {
+        const double a = 2.0 * M_PI * m / N;
+        t[m] = std::cos(a);
+        t[N + m] = -std::sin(a);
+        t[2 * N + m] = std::sin(M_PI * m / N);
+    }
+    return t;
+}
+
+// lambda_k = 4/h^2 sin^2(k pi / (2(n+1))), k = 1..n, computed in double and
+// rounded to the plan's element type like the reference's init_lm
+// (src/lapl_cube.cpp:145-172, src/lapl_rect.cpp:13-22); mode index k-1.
+std::vector<double> dirichlet_lambda(int n, double h, bool as_float) {
+    std::vector<double> l(n);
+    for (int k = 1; k <= n; k++) {
+        const double s = std::sin(k * M_PI * 0.5 / (n + 1));
+        const double v = 4. / (h * h) * (s * s);
+        l[k - 1] = as_float ? (double)(float)v : v;
+    }
+    return l;
+}
+
+bool valid_precision(int p) { return p == FDMGPU_F32 || p == FDMGPU_F64 || p == FDMGPU_F32_REFINE64; }
+
+int alloc_tables(fdmgpu_plan_t* P) {
+    const bool f32 = P->precision != FDMGPU_F64;
+    const std::vector<double> tx = dst_tables(1 << P->logx);
+    int rc = f32 ? upload<float>(&P->tab_x, tx) : upload<double>(&P->tab_x, tx);
+    if (rc) return rc;
+    if (P->logy == P->logx) {
+        P->tab_y = P->tab_x;
+    } else {
+        const std::vector<double> ty = dst_tables(1 << P->logy);
+        rc = f32 ? upload<float>(&P->tab_y, ty) : upload<double>(&P->tab_y, ty);
+        if (rc) return rc;
+    }
+    return FDMGPU_OK;
+}
+
+int ensure_refine(fdmgpu_plan_t* P, int count) {
+    if (P->cap >= count && P->r32) return FDMGPU_OK;
+    hipFree(P->r32);
+    hipFree(P->partial);
+    hipFree(P->done);
+    hipFree(P->iters);
+    hipFree(P->resid);
+    P->r32 = nullptr;
+    P->partial = nullptr;
+    P->done = P->iters = nullptr;
+    P->resid = nullptr;
+    P->cap = 0;
+    const long elems = (long)P->nx * P->ny * P->nz;
+    const long nblk = residual_partials(P->nx, P->ny, P->nz);
+    HIPCHK(hipMalloc(&P->r32, sizeof(float) * elems * count));
+    HIPCHK(hipMalloc(&P->partial, sizeof(double) * 2 * nblk * count));
+    HIPCHK(hipMalloc(&P->done, sizeof(int) * count));
+    HIPCHK(hipMalloc(&P->iters, sizeof(int) * count));
+    HIPCHK(hipMalloc(&P->resid, sizeof(double) * count));
+    P->cap = count;
+    return FDMGPU_OK;
+}
+
+// Records HIP events around one launch when timing is enabled.
+struct Timed {
+    fdmgpu_plan_t* P;
+    size_t slot = 0;
+    hipStream_t s;
+    bool on;
+    Timed(fdmgpu_plan_t* P_, int k, hipStream_t s_) : P(P_), s(s_), on(P_->timing) {
+        if (!on) return;
+        slot = P->nrec++;
+        while (P->evs.size() < 2 * P->nrec) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) { on = false; P->nrec--; return; }
+            P->evs.push_back(e);
+        }
+        if (P->rec_kernel.size() < P->nrec) P->rec_kernel.resize(P->nrec);
+        P->rec_kernel[slot] = k;
+        (void)hipEventRecord(P->evs[2 * slot], s);
+    }
+    ~Timed() {
+        if (on) (void)hipEventRecord(P->evs[2 * slot + 1], s);
+    }
+};
+
+int harvest_timing(fdmgpu_plan_t* P) {
+    if (P->nrec == 0) return FDMGPU_OK;
+    HIPCHK(hipEventSynchronize(P->evs[2 * (P->nrec - 1) + 1]));
+    for (size_t i = 0; i < P->nrec; i++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, P->evs[2 * i], P->evs[2 * i + 1]));
+        P->tot_ms[P->rec_kernel[i]] += ms;
+        P->launches[P->rec_kernel[i]] += 1;
+    }
+    P->nrec = 0;
+    return FDMGPU_OK;
+}
+
+// ---- the three stages, templated on compute type T ------------------------
+// Forward xy transforms: in (Tin) -> out (T).
+template <typename Tin, typename T>
+int stage_fwd(fdmgpu_plan_t* P, const void* in, void* out, int count, hipStream_t s, const int* done) {
+    const long elems = (long)P->nx * P->ny * P->nz;
+    const double sx = P->dx * std::sqrt(2. / P->lx), sy = P->dy * std::sqrt(2. / P->ly);
+    if (P->kind == 3 && P->logx == P->logy && plane_fits<T>(P->logx)) {
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done};
+        Timed tm(P, FDMGPU_K_XY_FWD, s);
+        HIPCHK((launch_plane<Tin, T, T>(P->logx, a)));
+        return FDMGPU_OK;
+    }
+    if (P->kind == 3) {
+        AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, false, done};
+        {
+            Timed tm(P, FDMGPU_K_X_TILE, s);
+            HIPCHK((launch_xaxis<Tin, T, T>(P->logx, ax)));
+        }
+        AxisArgs ay{out, out, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done};
+        Timed tm(P, FDMGPU_K_Y_TILE, s);
+        HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
+        return FDMGPU_OK;
+    }
+    // rect: y only
+    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done};
+    Timed tm(P, FDMGPU_K_Y_TILE, s);
+    HIPCHK((launch_yaxis<Tin, T, T>(P->logy, ay)));
+    return FDMGPU_OK;
+}
+
+// Inverse xy transforms: in (T) -> out (Tout), optionally accumulating.
+template <typename T, typename Tout>
+int stage_inv(fdmgpu_plan_t* P, void* in, void* out, int count, hipStream_t s, bool accum, const int* done) {
+    const long elems = (long)P->nx * P->ny * P->nz;
+    const double sx = std::sqrt(2. / P->lx), sy = std::sqrt(2. / P->ly);
+    if (P->kind == 3 && P->logx == P->logy && plane_fits<T>(P->logx)) {
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done};
+        Timed tm(P, FDMGPU_K_XY_INV, s);
+        HIPCHK((launch_plane<T, Tout, T>(P->logx, a)));
+        return FDMGPU_OK;
+    }
+    if (P->kind == 3) {
+        AxisArgs ay{in, in, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done};
+        {
+            Timed tm(P, FDMGPU_K_Y_TILE, s);
+            HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
+        }
+        AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, accum, done};
+        Timed tm(P, FDMGPU_K_X_TILE, s);
+        HIPCHK((launch_xaxis<T, Tout, T>(P->logx, ax)));
+        return FDMGPU_OK;
+    }
+    if (accum) {   // rect: inverse y into the FP32 buffer, then accumulate through an x pass-free add
+        AxisArgs ay{in, in, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done};
+        Timed tm(P, FDMGPU_K_Y_TILE, s);
+        HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
+        return -1;   // caller adds
+    }
+    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done};
+    Timed tm(P, FDMGPU_K_Y_TILE, s);
+    HIPCHK((launch_yaxis<T, Tout, T>(P->logy, ay)));
+    return FDMGPU_OK;
+}
+
+template <typename T>
+int stage_tridiag(fdmgpu_plan_t* P, void* data, int count, hipStream_t s, const int* done) {
+    const long elems = (long)P->nx * P->ny * P->nz;
+    if (P->kind == 3) {
+        const double b = 1.0 / (P->hz_op * P->hz_op);
+        const double gamma = P->dz * (double)(1 << P->logz) / P->lz;
+        ZcrArgs z{data, (long)P->nx * P->ny, P->nx, P->lmx, P->lmy, b, -2.0 * b,
+                  std::fabs(gamma - 1.0) < 4e-16 ? 1.0 : gamma, elems, count, s, done};
+        Timed tm(P, FDMGPU_K_ZCR, s);
+        HIPCHK(launch_zcr<T>(P->logz, z));
+        return FDMGPU_OK;
+    }
+    RowCrArgs r{data, P->nx, P->ny, P->lmy, P->d_sc, P->d_Ls, P->d_Us, 1.0 / (P->dx * P->dx), elems, count, s, done};
+    Timed tm(P, FDMGPU_K_ROWCR, s);
+    HIPCHK(launch_rowcr<T>(r));
+    return FDMGPU_OK;
+}
+
+template <typename T>
+int solve_plain(fdmgpu_plan_t* P, void* ans, const void* rhs, int count, hipStream_t s) {
+    int rc = stage_fwd<T, T>(P, rhs, ans, count, s, nullptr);
+    if (rc) return rc;
+    rc = stage_tridiag<T>(P, ans, count, s, nullptr);
+    if (rc) return rc;
+    return stage_inv<T, T>(P, ans, ans, count, s, false, nullptr);
+}
+
+__global__ void add_f32_to_f64(double* __restrict__ u, const float* __restrict__ c, const long n,
+                               const long bs, const int* __restrict__ done) {
+    if (done && done[blockIdx.y]) return;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[blockIdx.y * bs + i] += (double)c[blockIdx.y * bs + i];
+}
+
+// FP32 inner solves + FP64 residual/update/norm.  Iteration i:
+//   c = solve32(r_{i-1}) (r_0 = f), u += c, r_i = f - A u, stop when
+//   ||r_i||/||f|| <= rtol (device flag per problem; later launches skip it).
+int solve_refine(fdmgpu_plan_t* P, double* ans, const double* rhs, int count, hipStream_t s) {
+    int rc = ensure_refine(P, count);
+    if (rc) return rc;
+    const long elems = (long)P->nx * P->ny * P->nz;
+    if ((const void*)ans == (const void*)rhs) {   // f is needed by every residual: keep a copy
+        const size_t bytes = sizeof(double) * elems * count;
+        if (P->fcopy_bytes < bytes) {
+            hipFree(P->fcopy);
+            P->fcopy = nullptr;
+            P->fcopy_bytes = 0;
+            HIPCHK(hipMalloc(&P->fcopy, bytes));
+            P->fcopy_bytes = bytes;
+        }
+        Timed tm(P, FDMGPU_K_OTHER, s);
+        HIPCHK(hipMemcpyAsync(P->fcopy, rhs, bytes, hipMemcpyDeviceToDevice, s));
+        rhs = P->fcopy;
+    }
+    {
+        Timed tm(P, FDMGPU_K_OTHER, s);
+        HIPCHK(launch_fill_int(P->done, count, 0, s));
+        HIPCHK(launch_fill_int(P->iters, count, 0, s));
+    }
+    ResidualArgs ra{};
+    ra.f = rhs;
+    ra.u = ans;
+    ra.r32 = P->r32;
+    ra.partial = P->partial;
+    ra.nx = P->nx;
+    ra.ny = P->ny;
+    ra.nz = P->nz;
+    ra.three = P->kind == 3;
+    ra.ix = 1.0 / (P->hx_op * P->hx_op);
+    ra.iy = 1.0 / (P->dy * P->dy);
+    ra.iz = P->kind == 3 ? 1.0 / (P->hz_op * P->hz_op) : 0.0;
+    ra.sy = P->scaled ? P->d_sc64 : nullptr;
+    ra.lx = P->d_Ls64;
+    ra.ux = P->d_Us64;
+    ra.bs = elems;
+    ra.batch = count;
+    ra.tol = P->rtol;
+    ra.done = P->done;
+    ra.iters = P->iters;
+    ra.resid = P->resid;
+    ra.stream = s;
+    for (int it = 1; it <= P->max_iter; ++it) {
+        if (it == 1) rc = stage_fwd<double, float>(P, rhs, P->r32, count, s, P->done);
+        else rc = stage_fwd<float, float>(P, P->r32, P->r32, count, s, P->done);
+        if (rc) return rc;
+        rc = stage_tridiag<float>(P, P->r32, count, s, P->done);
+        if (rc) return rc;
+        rc = stage_inv<float, double>(P, P->r32, ans, count, s, it > 1, P->done);
+        if (rc == -1) {   // rect accumulate
+            dim3 g((unsigned)((elems + 255) / 256), count);
+            Timed tm(P, FDMGPU_K_OTHER, s);
+            hipLaunchKernelGGL(add_f32_to_f64, g, dim3(256), 0, s, ans, (const float*)P->r32, elems, elems,
+                               (const int*)P->done);
+            HIPCHK(hipGetLastError());
+            rc = 0;
+        }
+        if (rc) return rc;
+        ra.it = it;
+        Timed tm(P, FDMGPU_K_RESIDUAL, s);
+        HIPCHK(launch_residual(ra));
+    }
+    return FDMGPU_OK;
+}
+
+int solve_device_impl(fdmgpu_plan_t* P, void* ans, const void* rhs, int count, hipStream_t s) {
+    HIPCHK(hipSetDevice(P->device));
+    P->last_stream = s;
+    P->last_count = count;
+    if (count <= 0) return FDMGPU_OK;
+    switch (P->precision) {
+        case FDMGPU_F32: return solve_plain<float>(P, ans, rhs, count, s);
+        case FDMGPU_F64: return solve_plain<double>(P, ans, rhs, count, s);
+        default: return solve_refine(P, (double*)ans, (const double*)rhs, count, s);
+    }
+}
+
+int common_init(fdmgpu_plan_t* P, int precision) {
+    P->precision = precision;
+    P->elem = precision == FDMGPU_F32 ? 4 : 8;
+    P->csize = precision == FDMGPU_F64 ? 8 : 4;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) {
+        std::snprintf(g_errbuf, sizeof g_errbuf, "no HIP device available");
+        return FDMGPU_ERR_DEVICE;
+    }
+    HIPCHK(hipGetDevice(&P->device));
+    return FDMGPU_OK;
+}
+
+void free_plan(fdmgpu_plan_t* P) {
+    if (!P) return;
+    if (P->tab_y != P->tab_x) hipFree(P->tab_y);
+    hipFree(P->tab_x);
+    hipFree(P->lmx);
+    hipFree(P->lmy);
+    hipFree(P->d_sc);
+    hipFree(P->d_Ls);
+    hipFree(P->d_Us);
+    hipFree(P->d_sc64);
+    hipFree(P->d_Ls64);
+    hipFree(P->d_Us64);
+    hipFree(P->r32);
+    hipFree(P->partial);
+    hipFree(P->done);
+    hipFree(P->iters);
+    hipFree(P->resid);
+    hipFree(P->stage);
+    hipFree(P->fcopy);
+    for (hipEvent_t e : P->evs) (void)hipEventDestroy(e);
+    if (P->own) hipStreamDestroy(P->own);
+    delete P;
+}
+
+int set_scales_impl(fdmgpu_plan_t* P, const double* s, const double* L, const double* U) {
+    const int nx = P->nx;
+    P->h_sc.assign(nx, 1.0);
+    P->h_Ls.assign(nx, 1.0);
+    P->h_Us.assign(nx, 1.0);
+    for (int j = 0; j < nx; j++) {
+        if (s) P->h_sc[j] = s[j + 1];
+        if (L) P->h_Ls[j] = L[j + 1];
+        if (U) P->h_Us[j] = U[j + 1];
+    }
+    if (P->precision == FDMGPU_F32) {   // LaplRect<float> stores the scales as float
+        for (int j = 0; j < nx; j++) {
+            P->h_sc[j] = (double)(float)P->h_sc[j];
+            P->h_Ls[j] = (double)(float)P->h_Ls[j];
+            P->h_Us[j] = (double)(float)P->h_Us[j];
+        }
+    }
+    // the GPU eliminates without row interchanges: require diagonal dominance of
+    // every row system (the default scales and the cylindrical ones satisfy it)
+    std::vector<double> lam = dirichlet_lambda(P->ny, P->dy, false);
+    const double i2 = 1.0 / (P->dx * P->dx);
+    for (int k = 0; k < P->ny; k++) {
+        for (int j = 0; j < nx; j++) {
+            const double d = std::fabs(-2.0 * i2 - lam[k] * P->h_sc[j]);
+            const double off = (j > 0 ? std::fabs(P->h_Ls[j]) : 0.0) * i2 + (j < nx - 1 ? std::fabs(P->h_Us[j]) : 0.0) * i2;
+            if (!(d >= off) || d == 0.0) {
+                std::snprintf(g_errbuf, sizeof g_errbuf,
+                              "LaplRect row %d column %d is not diagonally dominant: the GPU tridiagonal solve "
+                              "does not pivot",
+                              k + 1, j + 1);
+                return FDMGPU_ERR_SINGULAR;
+            }
+        }
+    }
+    const bool f32 = P->precision != FDMGPU_F64;
+    int rc;
+    if ((rc = f32 ? upload<float>(&P->d_sc, P->h_sc) : upload<double>(&P->d_sc, P->h_sc))) return rc;
+    if ((rc = f32 ? upload<float>(&P->d_Ls, P->h_Ls) : upload<double>(&P->d_Ls, P->h_Ls))) return rc;
+    if ((rc = f32 ? upload<float>(&P->d_Us, P->h_Us) : upload<double>(&P->d_Us, P->h_Us))) return rc;
+    if ((rc = upload<double>((void**)&P->d_sc64, P->h_sc))) return rc;
+    if ((rc = upload<double>((void**)&P->d_Ls64, P->h_Ls))) return rc;
+    if ((rc = upload<double>((void**)&P->d_Us64, P->h_Us))) return rc;
+    P->scaled = false;
+    for (int j = 0; j < nx; j++)
+        if (P->h_sc[j] != 1.0 || P->h_Ls[j] != 1.0 || P->h_Us[j] != 1.0) P->scaled = true;
+    return FDMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdmgpu_version(void) { return 100; }
+
+const char* fdmgpu_strerror(int code) {
+    switch (code) {
+        case FDMGPU_OK: return "ok";
+        case FDMGPU_ERR_INVALID_SIZE: return "transform length must be a power of two (verify((1<<n)==N))";
+        case FDMGPU_ERR_INVALID_ARG: return "invalid argument";
+        case FDMGPU_ERR_UNSUPPORTED: return "configuration not supported on the GPU";
+        case FDMGPU_ERR_OUT_OF_MEMORY: return "device out of memory";
+        case FDMGPU_ERR_DEVICE: return g_errbuf[0] ? g_errbuf : "HIP device error";
+        case FDMGPU_ERR_SINGULAR: return g_errbuf[0] ? g_errbuf : "tridiagonal system needs pivoting (gtsv info != 0)";
+        case FDMGPU_ERR_NOT_CONVERGED: return "refinement did not reach the requested residual";
+        case FDMGPU_ERR_ADMISSION: return "request exceeds the HBM budget";
+        default: return "unknown error";
+    }
+}
+
+int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double dz,
+                     double lx, double ly, double lz, int nx, int ny, int nz) {
+    if (!plan || !valid_precision(precision)) return FDMGPU_ERR_INVALID_ARG;
+    *plan = nullptr;
+    if (periodic_mask != 0 && periodic_mask != 7) return FDMGPU_ERR_INVALID_ARG;
+    if (nx < 1 || ny < 1 || nz < 1) return FDMGPU_ERR_INVALID_SIZE;
+    if (periodic_mask) return FDMGPU_ERR_UNSUPPORTED;
+    const int lgx = ilog2_exact(nx + 1), lgy = ilog2_exact(ny + 1), lgz = ilog2_exact(nz + 1);
+    if (lgx < 0 || lgy < 0 || lgz < 0) return FDMGPU_ERR_INVALID_SIZE;
+    if (lgx > 9 || lgy > 9 || lgz > 9 || lgx < 1 || lgy < 1 || lgz < 1) return FDMGPU_ERR_UNSUPPORTED;
+    fdmgpu_plan_t* P = new (std::nothrow) fdmgpu_plan_t();
+    if (!P) return FDMGPU_ERR_OUT_OF_MEMORY;
+    int rc = common_init(P, precision);
+    if (rc) { free_plan(P); return rc; }
+    P->kind = 3;
+    P->nx = nx; P->ny = ny; P->nz = nz;
+    P->logx = lgx; P->logy = lgy; P->logz = lgz;
+    P->dx = dx; P->dy = dy; P->dz = dz; P->lx = lx; P->ly = ly; P->lz = lz;
+    if (precision == FDMGPU_F32_REFINE64) {
+        // refinement converges to A^{-1} f; the reference returns g A^{-1} f with
+        // g = prod_a h_a N_a / l_a, which is 1 for the usual l = (n+1) h geometry
+        const double g = dx * (nx + 1) / lx * dy * (ny + 1) / ly * dz * (nz + 1) / lz;
+        if (std::fabs(g - 1.0) > 1e-13) { free_plan(P); return FDMGPU_ERR_UNSUPPORTED; }
+    }
+    // eigenvalue aliasing of the reference (src/lapl_cube.cpp:162,171)
+    P->hx_op = (nx == ny) ? dy : dx;
+    P->hz_op = (nz == ny) ? dy : dz;
+    const bool f32 = precision != FDMGPU_F64;
+    const bool tf = precision == FDMGPU_F32;   // the reference's lambda tables are stored as T
+    std::vector<double> lmy = dirichlet_lambda(ny, dy, tf);
+    std::vector<double> lmx = (nx == ny) ? lmy : dirichlet_lambda(nx, dx, tf);
+    if ((rc = alloc_tables(P)) ||
+        (rc = f32 ? upload<float>(&P->lmx, lmx) : upload<double>(&P->lmx, lmx)) ||
+        (rc = f32 ? upload<float>(&P->lmy, lmy) : upload<double>(&P->lmy, lmy))) {
+        free_plan(P);
+        return rc;
+    }
+    *plan = P;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double lx,
+                     double ly, int nx, int ny) {
+    if (!plan || !valid_precision(precision)) return FDMGPU_ERR_INVALID_ARG;
+    *plan = nullptr;
+    if (periodic_mask != 0 && periodic_mask != FDMGPU_PERIODIC_Y) return FDMGPU_ERR_INVALID_ARG;
+    if (nx < 1 || ny < 1) return FDMGPU_ERR_INVALID_SIZE;
+    if (periodic_mask) return FDMGPU_ERR_UNSUPPORTED;
+    const int lgy = ilog2_exact(ny + 1);
+    if (lgy < 0) return FDMGPU_ERR_INVALID_SIZE;
+    if (lgy < 1 || lgy > 9 || nx > 4096) return FDMGPU_ERR_UNSUPPORTED;   // row CR stages 4*nx values in LDS
+    fdmgpu_plan_t* P = new (std::nothrow) fdmgpu_plan_t();
+    if (!P) return FDMGPU_ERR_OUT_OF_MEMORY;
+    int rc = common_init(P, precision);
+    if (rc) { free_plan(P); return rc; }
+    P->kind = 2;
+    P->nx = nx; P->ny = ny; P->nz = 1;
+    P->logx = lgy; P->logy = lgy; P->logz = 0;
+    P->dx = dx; P->dy = dy; P->lx = lx; P->ly = ly;
+    P->hx_op = dx;
+    if (precision == FDMGPU_F32_REFINE64 && std::fabs(dy * (ny + 1) / ly - 1.0) > 1e-13) {
+        free_plan(P);
+        return FDMGPU_ERR_UNSUPPORTED;
+    }
+    const bool f32 = precision != FDMGPU_F64;
+    std::vector<double> lmy = dirichlet_lambda(ny, dy, precision == FDMGPU_F32);
+    if ((rc = alloc_tables(P)) || (rc = f32 ? upload<float>(&P->lmy, lmy) : upload<double>(&P->lmy, lmy)) ||
+        (rc = set_scales_impl(P, nullptr, nullptr, nullptr))) {
+        free_plan(P);
+        return rc;
+    }
+    *plan = P;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_rect_set_scales(fdmgpu_plan plan, const double* lm_y_scale, const double* L_scale,
+                           const double* U_scale) {
+    if (!plan || plan->kind != 2) return FDMGPU_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(plan->device));
+    return set_scales_impl(plan, lm_y_scale, L_scale, U_scale);
+}
+
+int fdmgpu_set_refine(fdmgpu_plan plan, double rel_tol, int max_iter) {
+    if (!plan || !(rel_tol > 0) || max_iter < 1 || max_iter > 64) return FDMGPU_ERR_INVALID_ARG;
+    plan->rtol = rel_tol;
+    plan->max_iter = max_iter;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_solve_device(fdmgpu_plan plan, void* d_ans, const void* d_rhs, void* stream) {
+    if (!plan || !d_ans || !d_rhs) return FDMGPU_ERR_INVALID_ARG;
+    return solve_device_impl(plan, d_ans, d_rhs, 1, (hipStream_t)stream);
+}
+
+int fdmgpu_solve_batch_device(fdmgpu_plan plan, int count, void* d_ans, const void* d_rhs, void* stream) {
+    if (!plan || count < 0 || (count > 0 && (!d_ans || !d_rhs))) return FDMGPU_ERR_INVALID_ARG;
+    if (count > 65535) return FDMGPU_ERR_UNSUPPORTED;
+    return solve_device_impl(plan, d_ans, d_rhs, count, (hipStream_t)stream);
+}
+
+int fdmgpu_solve_host(fdmgpu_plan plan, void* ans, const void* rhs) {
+    if (!plan || !ans || !rhs) return FDMGPU_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(plan->device));
+    const size_t bytes = (size_t)plan->nx * plan->ny * plan->nz * plan->elem;
+    const size_t need = plan->precision == FDMGPU_F32_REFINE64 ? 2 * bytes : bytes;
+    if (plan->stage_bytes < need) {
+        hipFree(plan->stage);
+        plan->stage = nullptr;
+        plan->stage_bytes = 0;
+        HIPCHK(hipMalloc(&plan->stage, need));
+        plan->stage_bytes = need;
+    }
+    if (!plan->own) HIPCHK(hipStreamCreateWithFlags(&plan->own, hipStreamNonBlocking));
+    char* d_rhs = (char*)plan->stage;
+    char* d_ans = plan->precision == FDMGPU_F32_REFINE64 ? d_rhs + bytes : d_rhs;
+    HIPCHK(hipMemcpyAsync(d_rhs, rhs, bytes, hipMemcpyHostToDevice, plan->own));
+    int rc = solve_device_impl(plan, d_ans, d_rhs, 1, plan->own);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ans, d_ans, bytes, hipMemcpyDeviceToHost, plan->own));
+    HIPCHK(hipStreamSynchronize(plan->own));
+    if (plan->precision == FDMGPU_F32_REFINE64) {
+        fdmgpu_stats st;
+        rc = fdmgpu_get_stats(plan, &st);
+        if (rc) return rc;
+        if (!(st.rel_residual <= plan->rtol)) return FDMGPU_ERR_NOT_CONVERGED;
+    }
+    return FDMGPU_OK;
+}
+
+int fdmgpu_get_stats(fdmgpu_plan plan, fdmgpu_stats* st) {
+    if (!plan || !st) return FDMGPU_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(plan->device));
+    HIPCHK(hipStreamSynchronize(plan->last_stream));
+    st->iterations = 1;
+    st->rel_residual = -1.0;
+    st->max_rel_residual = -1.0;
+    if (plan->precision == FDMGPU_F32_REFINE64 && plan->last_count > 0) {
+        const int c = plan->last_count;
+        std::vector<int> it(c);
+        std::vector<double> r(c);
+        HIPCHK(hipMemcpy(it.data(), plan->iters, sizeof(int) * c, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(r.data(), plan->resid, sizeof(double) * c, hipMemcpyDeviceToHost));
+        st->iterations = *std::max_element(it.begin(), it.end());
+        st->rel_residual = r[c - 1];
+        st->max_rel_residual = *std::max_element(r.begin(), r.end());
+    }
+    return FDMGPU_OK;
+}
+
+int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resident,
+                          unsigned long long* per_solve) {
+    if (!plan || count < 0) return FDMGPU_ERR_INVALID_ARG;
+    const unsigned long long elems = (unsigned long long)plan->nx * plan->ny * plan->nz;
+    const unsigned long long tabs = 3ull * ((1ull << plan->logx) + (1ull << plan->logy)) * plan->csize +
+                                    (unsigned long long)(plan->nx + plan->ny) * plan->csize;
+    if (resident) *resident = tabs;
+    if (per_solve) {
+        unsigned long long s = 0;
+        if (plan->precision == FDMGPU_F32_REFINE64)
+            s = count * (elems * 4 + 16ull * residual_partials(plan->nx, plan->ny, plan->nz) + 16);
+        *per_solve = s;
+    }
+    return FDMGPU_OK;
+}
+
+int fdmgpu_set_timing(fdmgpu_plan plan, int enable) {
+    if (!plan) return FDMGPU_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(plan->device));
+    int rc = harvest_timing(plan);
+    if (rc) return rc;
+    plan->timing = enable != 0;
+    for (int k = 0; k < FDMGPU_NUM_KERNELS; k++) {
+        plan->tot_ms[k] = 0;
+        plan->launches[k] = 0;
+    }
+    return FDMGPU_OK;
+}
+
+int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launches) {
+    if (!plan || kernel < 0 || kernel >= FDMGPU_NUM_KERNELS) return FDMGPU_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(plan->device));
+    int rc = harvest_timing(plan);
+    if (rc) return rc;
+    if (total_ms) *total_ms = plan->tot_ms[kernel];
+    if (launches) *launches = plan->launches[kernel];
+    return FDMGPU_OK;
+}
+
+int fdmgpu_destroy(fdmgpu_plan plan) {
+    if (!plan) return FDMGPU_ERR_INVALID_ARG;
+    hipSetDevice(plan->device);
+    if (plan->last_stream || plan->own) hipStreamSynchronize(plan->last_stream);
+    free_plan(plan);
+    return FDMGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,496 @@
+// HIP kernels for the fast Poisson solve (gfx950) and their host launchers.
+//
+// Data layout in HBM: a field is dense [nz][ny][nx] (x fastest), interior
+// points only, exactly the array LaplCube::solve receives (reference
+// src/lapl_cube.h:80, src/tensor.h:192-205).  Batches of equal-shape problems
+// are laid out back to back (batch stride = nx*ny*nz elements).
+//
+// Kernels:
+//   plane_dst_kernel   DST-I along x and y of whole z-planes held in LDS
+//                      (square planes, N <= 128): one HBM read + one write.
+//   xaxis_dst_kernel   DST-I along x of a tile of TILE rows (any N <= 256).
+//   yaxis_dst_kernel   DST-I along y of a tile of TILE columns of one plane.
+//   zcr_kernel         constant-coefficient cyclic reduction along z for every
+//                      (kx,ky) mode (the tridiagonal replacing the reference's
+//                      z transform pair + divide).
+//   rowcr_kernel       general-n cyclic reduction along x with per-row
+//                      coefficients (LaplRect's gtsv step).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dst_device.hpp"
+#include "kernels.hpp"
+
+namespace fdmgpu {
+
+// ---------------------------------------------------------------------------
+// plane kernel
+// ---------------------------------------------------------------------------
+template <int LOGN>
+struct PlaneCfg {
+    using S = DstShape<LOGN>;
+    static constexpr int N = S::N;
+    static constexpr int P = N + S::PAD;
+    static constexpr int WORK = (N / 2) * S::N1;
+    static constexpr int NT = WORK < 64 ? 64 : WORK;
+    template <typename T>
+    static constexpr size_t lds_bytes() {
+        return sizeof(T) * ((size_t)N * P + 3 * N);
+    }
+};
+
+template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM>
+__global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_dst_kernel(
+    const Tin* __restrict__ in, Tout* __restrict__ out, const long bs_in, const long bs_out, const T sx,
+    const T sy, const T* __restrict__ tables, const int* __restrict__ done) {
+    using C = PlaneCfg<LOGN>;
+    if (done && done[blockIdx.y]) return;
+    constexpr int N = C::N, n = N - 1, P = C::P, NT = C::NT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    T* tab = lds + N * P;
+    const int tid = threadIdx.x;
+    const long plane_off = (long)blockIdx.x * n * n;
+    const Tin* src = in + blockIdx.y * bs_in + plane_off;
+    Tout* dst = out + blockIdx.y * bs_out + plane_off;
+
+    for (int i = tid; i < 3 * N; i += NT) tab[i] = tables[i];
+    for (int i = tid; i < N; i += NT) {
+        lds[i] = T(0);       // dummy row 0
+        lds[i * P] = T(0);   // dummy column 0
+    }
+    for (int idx = tid; idx < n * n; idx += NT) {
+        const int y = idx / n, x = idx - y * n;
+        lds[(y + 1) * P + x + 1] = (T)src[idx];
+    }
+    __syncthreads();
+    const DstTables<T> tb{tab, tab + N, tab + 2 * N};
+    dst_pairs<T, LOGN, N / 2>(lds, P, 1, tb, sx, tid, NT);   // pencils = rows  (x)
+    dst_pairs<T, LOGN, N / 2>(lds, 1, P, tb, sy, tid, NT);   // pencils = columns (y)
+    for (int idx = tid; idx < n * n; idx += NT) {
+        const int y = idx / n, x = idx - y * n;
+        if constexpr (ACCUM) dst[idx] += (Tout)lds[(y + 1) * P + x + 1];
+        else dst[idx] = (Tout)lds[(y + 1) * P + x + 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// axis-tile kernels (planes that do not fit LDS, or non-square planes)
+// ---------------------------------------------------------------------------
+template <int LOGN>
+struct TileCfg {
+    using S = DstShape<LOGN>;
+    static constexpr int N = S::N;
+    static constexpr int TILE = 32;                 // pencils per block (even)
+    static constexpr int P = N + S::PAD;            // x tiles: pencil stride
+    static constexpr int SJ = TILE + 64 / S::N1;    // y tiles: slot stride
+    static constexpr int WORK = (TILE / 2) * S::N1;
+    static constexpr int NT = WORK < 64 ? 64 : WORK;
+    template <typename T>
+    static constexpr size_t lds_x() { return sizeof(T) * ((size_t)TILE * P + 3 * N); }
+    template <typename T>
+    static constexpr size_t lds_y() { return sizeof(T) * ((size_t)N * SJ + 3 * N); }
+};
+
+// rows = nrows pencils of length n = N-1, contiguous; batch via blockIdx.y.
+template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM>
+__global__ __launch_bounds__(TileCfg<LOGN>::NT) void xaxis_dst_kernel(
+    const Tin* __restrict__ in, Tout* __restrict__ out, const long nrows, const long bs_in, const long bs_out,
+    const T scale, const T* __restrict__ tables, const int* __restrict__ done) {
+    using C = TileCfg<LOGN>;
+    if (done && done[blockIdx.y]) return;
+    constexpr int N = C::N, n = N - 1, P = C::P, NT = C::NT, TILE = C::TILE;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    T* tab = lds + TILE * P;
+    const int tid = threadIdx.x;
+    const long r0 = (long)blockIdx.x * TILE;
+    const Tin* src = in + blockIdx.y * bs_in + r0 * n;
+    Tout* dst = out + blockIdx.y * bs_out + r0 * n;
+    const int rows_here = (int)((nrows - r0) < TILE ? (nrows - r0) : TILE);
+
+    for (int i = tid; i < 3 * N; i += NT) tab[i] = tables[i];
+    for (int idx = tid; idx < TILE * n; idx += NT) {
+        const int p = idx / n, x = idx - p * n;
+        lds[p * P + x + 1] = p < rows_here ? (T)src[idx] : T(0);
+    }
+    __syncthreads();
+    const DstTables<T> tb{tab, tab + N, tab + 2 * N};
+    dst_pairs<T, LOGN, TILE / 2>(lds, P, 1, tb, scale, tid, NT);
+    for (int idx = tid; idx < rows_here * n; idx += NT) {
+        const int p = idx / n, x = idx - p * n;
+        if constexpr (ACCUM) dst[idx] += (Tout)lds[p * P + x + 1];
+        else dst[idx] = (Tout)lds[p * P + x + 1];
+    }
+}
+
+// Columns of plane blockIdx.y (nplanes planes of [n][nx]); batch via blockIdx.z.
+template <typename Tin, typename Tout, typename T, int LOGN>
+__global__ __launch_bounds__(TileCfg<LOGN>::NT) void yaxis_dst_kernel(
+    const Tin* __restrict__ in, Tout* __restrict__ out, const int nx, const long bs_in, const long bs_out,
+    const T scale, const T* __restrict__ tables, const int* __restrict__ done) {
+    using C = TileCfg<LOGN>;
+    constexpr int N = C::N, n = N - 1, SJ = C::SJ, NT = C::NT, TILE = C::TILE;
+    if (done && done[blockIdx.z]) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    T* tab = lds + N * SJ;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TILE;
+    const long poff = (long)blockIdx.y * n * nx;
+    const Tin* src = in + blockIdx.z * bs_in + poff;
+    Tout* dst = out + blockIdx.z * bs_out + poff;
+
+    for (int i = tid; i < 3 * N; i += NT) tab[i] = tables[i];
+    for (int idx = tid; idx < n * TILE; idx += NT) {
+        const int y = idx / TILE, p = idx - y * TILE;
+        const int x = x0 + p;
+        lds[(y + 1) * SJ + p] = x < nx ? (T)src[(long)y * nx + x] : T(0);
+    }
+    __syncthreads();
+    const DstTables<T> tb{tab, tab + N, tab + 2 * N};
+    dst_pairs<T, LOGN, TILE / 2>(lds, 1, SJ, tb, scale, tid, NT);
+    for (int idx = tid; idx < n * TILE; idx += NT) {
+        const int y = idx / TILE, p = idx - y * TILE;
+        const int x = x0 + p;
+        if (x < nx) dst[(long)y * nx + x] = (Tout)lds[(y + 1) * SJ + p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z cyclic reduction (see dst_device.hpp for the algorithm)
+// ---------------------------------------------------------------------------
+template <typename T, int Q>
+struct ZcrCfg {
+    static constexpr int NQ = 1 << Q;
+    static constexpr int G = NQ > 64 ? NQ / 64 : 1;   // lanes per system
+    static constexpr int C = NQ / G;                   // rows per lane (incl. separator slot)
+    static constexpr int W = 64 / G;                   // systems per wavefront
+    static constexpr int LOGG = ilog2(G);
+    static constexpr int LOC = G == 1 ? Q - 1 : Q - LOGG;   // levels local to a lane
+    static constexpr int NT = 256;
+};
+
+template <typename T, int Q>
+__global__ __launch_bounds__(256) void zcr_kernel(T* __restrict__ data, const long plane, const int nx,
+                                                  const T* __restrict__ lmx, const T* __restrict__ lmy,
+                                                  const T b, const T d0, const T gamma, const long bs,
+                                                  const int* __restrict__ done) {
+    using Cf = ZcrCfg<T, Q>;
+    if (done && done[blockIdx.y]) return;
+    constexpr int NQ = Cf::NQ, G = Cf::G, C = Cf::C, W = Cf::W, LOC = Cf::LOC, LOGG = Cf::LOGG;
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int s = lane % W, c = lane / W;
+    const long m = wave * W + s;
+    const bool valid = m < plane;
+    const long mm = valid ? m : 0;
+    T* base = data + blockIdx.y * bs + mm;
+    const int ky = (int)(mm / nx), kx = (int)(mm - (long)ky * nx);
+
+    T v[C];
+#pragma unroll
+    for (int r = 0; r < C; ++r) {
+        const int row = c * C + r;
+        v[r] = (valid && row < NQ - 1) ? base[(long)row * plane] * gamma : T(0);
+    }
+    CrLevels<T, Q> L;
+    cr_levels<T, Q>(L, b, d0 - lmx[kx] - lmy[ky]);
+
+    // forward reduction inside the lane's chunk
+    T toRight = T(0), toLeft = T(0);
+    static_for<1, LOC + 1>([&](auto ll) {
+        constexpr int l = decltype(ll)::value;
+        constexpr int st = 1 << l, h = st >> 1;
+        const T a = L.alpha[l];
+#pragma unroll
+        for (int r = st - 1; r < C - 1; r += st) v[r] += a * (v[r - h] + v[r + h]);
+        if constexpr (G > 1) {
+            toRight += a * v[C - 1 - h];
+            toLeft += a * v[h - 1];
+        }
+    });
+
+    T xleft = T(0);
+    if constexpr (G == 1) {
+        v[C / 2 - 1] *= L.inv_d[Q - 1];
+        v[C - 1] = T(0);
+    } else {
+        // separator c (rows (c+1)*C-1) collects both neighbours' contributions
+        const T fromRight = __shfl(toLeft, (lane + W) & 63, 64);
+        const T sepv = (c < G - 1) ? v[C - 1] + toRight + fromRight : T(0);
+        T sp[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) sp[i] = __shfl(sepv, s + i * W, 64);
+        // remaining levels LOC+1..Q-1 over the G-1 separators (index i <-> row (i+1)C-1)
+        static_for<1, LOGG>([&](auto ll) {
+            constexpr int lp = decltype(ll)::value;
+            constexpr int st = 1 << lp, h = st >> 1;
+            const T a = L.alpha[LOC + lp];
+#pragma unroll
+            for (int i = st - 1; i < G - 1; i += st) sp[i] += a * (sp[i - h] + sp[i + h]);
+        });
+        sp[G / 2 - 1] *= L.inv_d[Q - 1];
+        static_for<1, LOGG>([&](auto ll) {
+            constexpr int lp = LOGG - decltype(ll)::value;   // LOGG-1 .. 1
+            constexpr int st = 1 << lp, h = st >> 1;
+            const T cc = L.c[LOC + lp - 1], id = L.inv_d[LOC + lp - 1];
+#pragma unroll
+            for (int i = h - 1; i < G - 1; i += st) {
+                const T left = i - h >= 0 ? sp[i - h] : T(0);
+                const T right = i + h < G - 1 ? sp[i + h] : T(0);
+                sp[i] = (sp[i] - cc * (left + right)) * id;
+            }
+        });
+        // this lane's boundary values
+        T xl = T(0), xr = T(0);
+#pragma unroll
+        for (int i = 0; i < G - 1; ++i) {
+            if (i == c - 1) xl = sp[i];
+            if (i == c) xr = sp[i];
+        }
+        xleft = xl;
+        v[C - 1] = xr;
+    }
+
+    // back substitution inside the chunk
+    static_for<0, LOC>([&](auto ll) {
+        constexpr int l = LOC - decltype(ll)::value;   // LOC .. 1
+        constexpr int st = 1 << l, h = st >> 1;
+        const T cc = L.c[l - 1], id = L.inv_d[l - 1];
+#pragma unroll
+        for (int r = h - 1; r < C - 1; r += st) {
+            const T left = r - h >= 0 ? v[r - h] : xleft;
+            v[r] = (v[r] - cc * (left + v[r + h])) * id;
+        }
+    });
+
+    if (valid) {
+#pragma unroll
+        for (int r = 0; r < C; ++r) {
+            const int row = c * C + r;
+            if (row < NQ - 1) base[(long)row * plane] = v[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LaplRect x-direction: general-n cyclic reduction per row with per-row
+// coefficients (reference: init_Mat src/lapl_rect.cpp:44-60 + gtsv :90;
+// algorithm: cyclic_reduction_general src/cyclic_reduction.h:59-107).
+//   D_j = -2/dx^2 - lmy[k]*s_j,  L_j = Ls_j/dx^2 (j>0),  U_j = Us_j/dx^2 (j<nx-1).
+// One block per row; coefficients and rhs staged in LDS.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void rowcr_kernel(T* __restrict__ data, const int nx, const int q,
+                                                    const T* __restrict__ lmy, const T* __restrict__ sc,
+                                                    const T* __restrict__ Ls, const T* __restrict__ Us,
+                                                    const double inv_dx2, const long bs,
+                                                    const int* __restrict__ done) {
+    if (done && done[blockIdx.y]) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* d = reinterpret_cast<T*>(smem_raw);
+    T* e = d + nx;
+    T* f = e + nx;
+    T* bb = f + nx;
+    const int k = blockIdx.x;
+    T* row = data + blockIdx.y * bs + (long)k * nx;
+    const T lk = lmy[k];
+    for (int j = threadIdx.x; j < nx; j += blockDim.x) {
+        d[j] = (T)(-2.0 * inv_dx2 - (double)(lk * sc[j]));
+        e[j] = j > 0 ? (T)(Ls[j] * inv_dx2) : T(0);
+        f[j] = j < nx - 1 ? (T)(Us[j] * inv_dx2) : T(0);
+        bb[j] = row[j];
+    }
+    __syncthreads();
+    for (int l = 1; l < q; ++l) {
+        const int st = 1 << l, h = st >> 1;
+        for (int j = st - 1 + threadIdx.x * st; j < nx; j += blockDim.x * st) {
+            const T alpha = -e[j] / d[j - h];
+            if (j + h < nx) {
+                const T gamma = -f[j] / d[j + h];
+                d[j] += alpha * f[j - h] + gamma * e[j + h];
+                bb[j] += alpha * bb[j - h] + gamma * bb[j + h];
+                e[j] = alpha * e[j - h];
+                f[j] = gamma * f[j + h];
+            } else {
+                d[j] += alpha * f[j - h];
+                bb[j] += alpha * bb[j - h];
+                e[j] = alpha * e[j - h];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int top = (1 << (q - 1)) - 1;
+        if (top > nx - 1) top = nx - 1;
+        bb[top] = bb[top] / d[top];
+    }
+    __syncthreads();
+    for (int l = q - 1; l > 0; --l) {
+        const int st = 1 << l, h = st >> 1;
+        for (int j = h - 1 + threadIdx.x * st; j < nx; j += blockDim.x * st) {
+            T acc = bb[j];
+            if (j - h >= 0) acc -= e[j] * bb[j - h];
+            if (j + h < nx) acc -= f[j] * bb[j + h];
+            bb[j] = acc / d[j];
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < nx; j += blockDim.x) row[j] = bb[j];
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+namespace {
+template <typename K>
+hipError_t allow_lds(K kernel, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+}  // namespace
+
+template <typename Tin, typename Tout, typename T, int LOGN>
+static hipError_t plane_t(const PlaneArgs& a) {
+    using C = PlaneCfg<LOGN>;
+    auto k = a.accum ? plane_dst_kernel<Tin, Tout, T, LOGN, true> : plane_dst_kernel<Tin, Tout, T, LOGN, false>;
+    const size_t lds = C::template lds_bytes<T>();
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(a.nplanes, a.batch), dim3(C::NT), lds, a.stream, (const Tin*)a.in, (Tout*)a.out,
+                       a.bs_in, a.bs_out, (T)a.sx, (T)a.sy, (const T*)a.tables, a.done);
+    return hipGetLastError();
+}
+
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_plane(int logn, const PlaneArgs& a) {
+    switch (logn) {
+        case 1: return plane_t<Tin, Tout, T, 1>(a);
+        case 2: return plane_t<Tin, Tout, T, 2>(a);
+        case 3: return plane_t<Tin, Tout, T, 3>(a);
+        case 4: return plane_t<Tin, Tout, T, 4>(a);
+        case 5: return plane_t<Tin, Tout, T, 5>(a);
+        case 6: return plane_t<Tin, Tout, T, 6>(a);
+        case 7: return plane_t<Tin, Tout, T, 7>(a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+bool plane_fits(int logn) {
+    if (logn < 1 || logn > 7) return false;
+    size_t N = (size_t)1 << logn, N1 = (size_t)1 << (logn / 2);
+    return sizeof(T) * (N * (N + N1) + 3 * N) <= 160 * 1024;
+}
+
+template <typename Tin, typename Tout, typename T, int LOGN>
+static hipError_t xaxis_t(const AxisArgs& a) {
+    using C = TileCfg<LOGN>;
+    auto k = a.accum ? xaxis_dst_kernel<Tin, Tout, T, LOGN, true> : xaxis_dst_kernel<Tin, Tout, T, LOGN, false>;
+    const size_t lds = C::template lds_x<T>();
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    const long tiles = (a.nrows + C::TILE - 1) / C::TILE;
+    hipLaunchKernelGGL(k, dim3((unsigned)tiles, a.batch), dim3(C::NT), lds, a.stream, (const Tin*)a.in,
+                       (Tout*)a.out, a.nrows, a.bs_in, a.bs_out, (T)a.scale, (const T*)a.tables, a.done);
+    return hipGetLastError();
+}
+
+template <typename Tin, typename Tout, typename T, int LOGN>
+static hipError_t yaxis_t(const AxisArgs& a) {
+    using C = TileCfg<LOGN>;
+    auto k = yaxis_dst_kernel<Tin, Tout, T, LOGN>;
+    const size_t lds = C::template lds_y<T>();
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    const int tiles = (a.nx + C::TILE - 1) / C::TILE;
+    hipLaunchKernelGGL(k, dim3(tiles, a.nplanes, a.batch), dim3(C::NT), lds, a.stream, (const Tin*)a.in,
+                       (Tout*)a.out, a.nx, a.bs_in, a.bs_out, (T)a.scale, (const T*)a.tables, a.done);
+    return hipGetLastError();
+}
+
+#define FDM_AXIS_SWITCH(FN)                      \
+    switch (logn) {                              \
+        case 1: return FN<Tin, Tout, T, 1>(a);   \
+        case 2: return FN<Tin, Tout, T, 2>(a);   \
+        case 3: return FN<Tin, Tout, T, 3>(a);   \
+        case 4: return FN<Tin, Tout, T, 4>(a);   \
+        case 5: return FN<Tin, Tout, T, 5>(a);   \
+        case 6: return FN<Tin, Tout, T, 6>(a);   \
+        case 7: return FN<Tin, Tout, T, 7>(a);   \
+        case 8: return FN<Tin, Tout, T, 8>(a);   \
+        case 9: return FN<Tin, Tout, T, 9>(a);   \
+        default: return hipErrorInvalidValue;    \
+    }
+
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_xaxis(int logn, const AxisArgs& a) {
+    FDM_AXIS_SWITCH(xaxis_t)
+}
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_yaxis(int logn, const AxisArgs& a) {
+    FDM_AXIS_SWITCH(yaxis_t)
+}
+
+template <typename T, int Q>
+static hipError_t zcr_t(const ZcrArgs& a) {
+    using C = ZcrCfg<T, Q>;
+    const long lanes = ((a.plane + C::W - 1) / C::W) * 64;
+    const unsigned blocks = (unsigned)((lanes + C::NT - 1) / C::NT);
+    hipLaunchKernelGGL((zcr_kernel<T, Q>), dim3(blocks, a.batch), dim3(C::NT), 0, a.stream, (T*)a.data, a.plane,
+                       a.nx, (const T*)a.lmx, (const T*)a.lmy, (T)a.b, (T)a.d0, (T)a.gamma, a.bs, a.done);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_zcr(int q, const ZcrArgs& a) {
+    switch (q) {
+        case 1: return zcr_t<T, 1>(a);
+        case 2: return zcr_t<T, 2>(a);
+        case 3: return zcr_t<T, 3>(a);
+        case 4: return zcr_t<T, 4>(a);
+        case 5: return zcr_t<T, 5>(a);
+        case 6: return zcr_t<T, 6>(a);
+        case 7: return zcr_t<T, 7>(a);
+        case 8: return zcr_t<T, 8>(a);
+        case 9: return zcr_t<T, 9>(a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_rowcr(const RowCrArgs& a) {
+    int q = 0;
+    while ((1 << q) < a.nx + 1) q++;
+    const size_t lds = 4 * sizeof(T) * (size_t)a.nx;
+    hipError_t e = allow_lds(rowcr_kernel<T>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(rowcr_kernel<T>, dim3(a.nrows, a.batch), dim3(256), lds, a.stream, (T*)a.data, a.nx, q,
+                       (const T*)a.lmy, (const T*)a.sc, (const T*)a.Ls, (const T*)a.Us, a.inv_dx2, a.bs, a.done);
+    return hipGetLastError();
+}
+
+// explicit instantiations used by the plan code
+template hipError_t launch_plane<float, float, float>(int, const PlaneArgs&);
+template hipError_t launch_plane<double, double, double>(int, const PlaneArgs&);
+template hipError_t launch_plane<double, float, float>(int, const PlaneArgs&);
+template hipError_t launch_plane<float, double, float>(int, const PlaneArgs&);
+template bool plane_fits<float>(int);
+template bool plane_fits<double>(int);
+template hipError_t launch_xaxis<float, float, float>(int, const AxisArgs&);
+template hipError_t launch_xaxis<double, double, double>(int, const AxisArgs&);
+template hipError_t launch_xaxis<double, float, float>(int, const AxisArgs&);
+template hipError_t launch_xaxis<float, double, float>(int, const AxisArgs&);
+template hipError_t launch_yaxis<float, float, float>(int, const AxisArgs&);
+template hipError_t launch_yaxis<double, double, double>(int, const AxisArgs&);
+template hipError_t launch_yaxis<double, float, float>(int, const AxisArgs&);
+template hipError_t launch_yaxis<float, double, float>(int, const AxisArgs&);
+template hipError_t launch_zcr<float>(int, const ZcrArgs&);
+template hipError_t launch_zcr<double>(int, const ZcrArgs&);
+template hipError_t launch_rowcr<float>(const RowCrArgs&);
+template hipError_t launch_rowcr<double>(const RowCrArgs&);
+
+}  // namespace fdmgpu

@@ -1,0 +1,74 @@
+// Host-side launch interface of the solve kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fdmgpu {
+
+struct PlaneArgs {
+    const void* in;
+    void* out;
+    long bs_in, bs_out;   // batch strides (elements)
+    int nplanes, batch;
+    double sx, sy;        // DST scale factors for the x and y passes
+    const void* tables;   // 3N values of the compute type
+    hipStream_t stream;
+    bool accum;           // out += result (refinement update) instead of out = result
+    const int* done;      // per batch item: skip when non-zero (may be null)
+};
+
+struct AxisArgs {
+    const void* in;
+    void* out;
+    long nrows;           // x: rows per batch item (ny*nz)
+    int nx, nplanes;      // y: row length and number of planes
+    long bs_in, bs_out;
+    int batch;
+    double scale;
+    const void* tables;
+    hipStream_t stream;
+    bool accum;           // x kernel only
+    const int* done;
+};
+
+struct ZcrArgs {
+    void* data;
+    long plane;           // nx*ny modes per z-plane
+    int nx;
+    const void* lmx;      // eigenvalues per mode index (compute type)
+    const void* lmy;
+    double b, d0, gamma;  // 1/hz^2, -2/hz^2, rhs scale
+    long bs;
+    int batch;
+    hipStream_t stream;
+    const int* done;
+};
+
+struct RowCrArgs {
+    void* data;
+    int nx, nrows;
+    const void* lmy;      // per row
+    const void* sc;       // lm_y_scale per column (compute type)
+    const void* Ls;
+    const void* Us;
+    double inv_dx2;
+    long bs;
+    int batch;
+    hipStream_t stream;
+    const int* done;
+};
+
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_plane(int logn, const PlaneArgs& a);
+template <typename T>
+bool plane_fits(int logn);
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_xaxis(int logn, const AxisArgs& a);
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_yaxis(int logn, const AxisArgs& a);
+template <typename T>
+hipError_t launch_zcr(int q, const ZcrArgs& a);
+template <typename T>
+hipError_t launch_rowcr(const RowCrArgs& a);
+
+}  // namespace fdmgpu

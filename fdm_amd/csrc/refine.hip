@@ -1,0 +1,145 @@
+// FP64 side of FP32+refinement: residual stencil, norms, convergence flags.
+//
+// The operator is the one the reference's solve inverts (the 7-point /
+// 5-point discrete Laplacian with zero Dirichlet boundary, spacings as the
+// reference's eigenvalue tables use them, src/lapl_cube.cpp:145-172,
+// src/lapl_rect.cpp:13-22, 44-60):
+//   (A u)_i = (u_{x-1} - 2u + u_{x+1}) * cx_i  (+ LaplRect scales on x)
+//           + (u_{y-1} - 2u + u_{y+1}) * iy * sy_x
+//           + (u_{z-1} - 2u + u_{z+1}) * iz          (cube only)
+// Norm partials are reduced in a fixed order (no atomics), so a problem's
+// residual and iteration count do not depend on what else is in the batch.
+#include <hip/hip_runtime.h>
+
+#include "refine.hpp"
+
+namespace fdmgpu {
+
+constexpr int kResBlock = 256;
+
+template <typename Tr>
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = a;
+    sh[kResBlock + tid] = b;
+    __syncthreads();
+#pragma unroll
+    for (int s = kResBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            sh[tid] += sh[tid + s];
+            sh[kResBlock + tid] += sh[kResBlock + tid + s];
+        }
+        __syncthreads();
+    }
+    a = sh[0];
+    b = sh[kResBlock];
+}
+
+// r = f - A u (FP64), r32 = (float) r; partial[p][blk] = (sum r^2, sum f^2).
+// Cube: one block per 256 consecutive points of [nz][ny][nx].
+template <bool THREE, bool SCALED>
+__global__ __launch_bounds__(kResBlock) void residual_kernel(
+    const double* __restrict__ f, const double* __restrict__ u, float* __restrict__ r32,
+    double* __restrict__ partial, const int nx, const int ny, const int nz, const double ix, const double iy,
+    const double iz, const double* __restrict__ sy, const double* __restrict__ lx,
+    const double* __restrict__ ux, const long bs, const int* __restrict__ done) {
+    __shared__ double sh[2 * kResBlock];
+    const int p = blockIdx.y;
+    if (done && done[p]) return;
+    const long elems = (long)nx * ny * nz;
+    const long idx = (long)blockIdx.x * kResBlock + threadIdx.x;
+    const double* fp = f + p * bs;
+    const double* up = u + p * bs;
+    double rr = 0.0, ff = 0.0;
+    if (idx < elems) {
+        const int x = (int)(idx % nx);
+        const long t = idx / nx;
+        const int y = (int)(t % ny);
+        const int z = (int)(t / ny);
+        const double uc = up[idx];
+        const double xm = x > 0 ? up[idx - 1] : 0.0, xp = x < nx - 1 ? up[idx + 1] : 0.0;
+        const double ym = y > 0 ? up[idx - nx] : 0.0, yp = y < ny - 1 ? up[idx + nx] : 0.0;
+        double au;
+        if constexpr (SCALED) {
+            au = (lx[x] * xm - 2.0 * uc + ux[x] * xp) * ix + (ym - 2.0 * uc + yp) * iy * sy[x];
+        } else {
+            au = (xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy;
+        }
+        if constexpr (THREE) {
+            const long pl = (long)nx * ny;
+            const double zm = z > 0 ? up[idx - pl] : 0.0, zp = z < nz - 1 ? up[idx + pl] : 0.0;
+            au += (zm - 2.0 * uc + zp) * iz;
+        }
+        const double fv = fp[idx];
+        const double r = fv - au;
+        r32[p * bs + idx] = (float)r;
+        rr = r * r;
+        ff = fv * fv;
+    }
+    block_sum2<double>(rr, ff, sh);
+    if (threadIdx.x == 0) {
+        partial[((long)p * gridDim.x + blockIdx.x) * 2 + 0] = rr;
+        partial[((long)p * gridDim.x + blockIdx.x) * 2 + 1] = ff;
+    }
+}
+
+// One block per problem: fixed-order sum of the partials, then the flag.
+__global__ __launch_bounds__(kResBlock) void check_kernel(const double* __restrict__ partial, const int nblk,
+                                                         const double tol, int* __restrict__ done,
+                                                         int* __restrict__ iters, double* __restrict__ resid,
+                                                         const int it) {
+    __shared__ double sh[2 * kResBlock];
+    const int p = blockIdx.x;
+    if (done[p]) return;
+    double rr = 0.0, ff = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += kResBlock) {
+        rr += partial[((long)p * nblk + i) * 2 + 0];
+        ff += partial[((long)p * nblk + i) * 2 + 1];
+    }
+    block_sum2<double>(rr, ff, sh);
+    if (threadIdx.x == 0) {
+        const double rel = ff > 0.0 ? sqrt(rr / ff) : sqrt(rr);
+        resid[p] = rel;
+        iters[p] = it;
+        if (rel <= tol) done[p] = 1;
+    }
+}
+
+__global__ void fill_int_kernel(int* __restrict__ a, const int n, const int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+hipError_t launch_residual(const ResidualArgs& a) {
+    const long elems = (long)a.nx * a.ny * a.nz;
+    const unsigned nblk = (unsigned)((elems + kResBlock - 1) / kResBlock);
+    dim3 grid(nblk, a.batch);
+    if (a.nz > 1 || a.three) {
+        hipLaunchKernelGGL((residual_kernel<true, false>), grid, dim3(kResBlock), 0, a.stream, a.f, a.u, a.r32,
+                           a.partial, a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, nullptr, nullptr, nullptr, a.bs, a.done);
+    } else if (a.sy) {
+        hipLaunchKernelGGL((residual_kernel<false, true>), grid, dim3(kResBlock), 0, a.stream, a.f, a.u, a.r32,
+                           a.partial, a.nx, a.ny, 1, a.ix, a.iy, 0.0, a.sy, a.lx, a.ux, a.bs, a.done);
+    } else {
+        hipLaunchKernelGGL((residual_kernel<false, false>), grid, dim3(kResBlock), 0, a.stream, a.f, a.u, a.r32,
+                           a.partial, a.nx, a.ny, 1, a.ix, a.iy, 0.0, nullptr, nullptr, nullptr, a.bs, a.done);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(check_kernel, dim3(a.batch), dim3(kResBlock), 0, a.stream, a.partial, (int)nblk, a.tol, a.done,
+                       a.iters, a.resid, a.it);
+    return hipGetLastError();
+}
+
+long residual_partials(int nx, int ny, int nz) {
+    const long elems = (long)nx * ny * nz;
+    return (elems + kResBlock - 1) / kResBlock;
+}
+
+hipError_t launch_fill_int(int* a, int n, int v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace fdmgpu

@@ -1,0 +1,33 @@
+// FP64 residual / norm / convergence kernels for FP32+refinement (refine.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fdmgpu {
+
+struct ResidualArgs {
+    const double* f;
+    const double* u;
+    float* r32;          // out: (float)(f - A u)
+    double* partial;     // [batch][residual_partials()][2]
+    int nx, ny, nz;
+    bool three;          // 7-point (cube) vs 5-point (rect)
+    double ix, iy, iz;   // 1/h^2 per axis of the inverted operator
+    const double* sy;    // rect scales per column (lm_y_scale, L_scale, U_scale) or null
+    const double* lx;
+    const double* ux;
+    long bs;
+    int batch;
+    double tol;
+    int* done;
+    int* iters;
+    double* resid;
+    int it;
+    hipStream_t stream;
+};
+
+hipError_t launch_residual(const ResidualArgs& a);
+long residual_partials(int nx, int ny, int nz);
+hipError_t launch_fill_int(int* a, int n, int v, hipStream_t s);
+
+}  // namespace fdmgpu

@@ -1,0 +1,138 @@
+/* fdmgpu: MI355X (gfx950) fast Poisson solve -- C ABI.
+ *
+ * Drop-in boundary for the fdm fast Poisson solvers.  Plain pointers, sizes
+ * and integer error codes; no C++ or torch types.  The header-only C++ shim
+ * include/fdm/lapl_cube.h / include/fdm/lapl_rect.h rebuilds the reference's
+ * templated class API (same constructors, same solve(ans, rhs)) on top of it.
+ *
+ * Array convention (every entry point): a field is dense, x fastest,
+ * [nz][ny][nx] for a cube and [ny][nx] for a rect, interior points only --
+ * the array the reference's solve(T* ans, T* rhs) receives (reference
+ * src/lapl_cube.h:80, src/lapl_rect.h:65-68, src/tensor.h:192-205).
+ * Element type: float for FDMGPU_F32 plans, double for FDMGPU_F64 and
+ * FDMGPU_F32_REFINE64 plans.
+ *
+ * Threading: a plan is not reentrant (the reference's solver object is not
+ * either: member scratch, src/lapl_cube.h:96-97); different plans may be used
+ * concurrently from different threads on different streams.
+ */
+#ifndef FDMGPU_H
+#define FDMGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdmgpu_plan_t* fdmgpu_plan;
+
+/* ---- error codes --------------------------------------------------------
+ * The reference has no return codes: it calls verify() -> fprintf + abort()
+ * (src/verify.h:1-36).  The C++ shim maps every non-zero code back to that
+ * behaviour. */
+#define FDMGPU_OK 0
+#define FDMGPU_ERR_INVALID_SIZE 1  /* transform length not a power of two: verify((1<<n)==N), src/fft.cpp:67,105 */
+#define FDMGPU_ERR_INVALID_ARG 2   /* null plan/pointer, bad precision or mask */
+#define FDMGPU_ERR_UNSUPPORTED 3   /* valid for the reference, not implemented on the GPU */
+#define FDMGPU_ERR_OUT_OF_MEMORY 4 /* device allocation failed */
+#define FDMGPU_ERR_DEVICE 5        /* HIP runtime error (no device, launch failure) */
+#define FDMGPU_ERR_SINGULAR 6      /* tridiagonal system not solvable without pivoting: gtsv info != 0, src/lapl_rect.cpp:91 */
+#define FDMGPU_ERR_NOT_CONVERGED 7 /* FP32+refine did not reach the requested residual */
+#define FDMGPU_ERR_ADMISSION 8     /* serving: request larger than the HBM budget */
+
+/* ---- precision -----------------------------------------------------------*/
+#define FDMGPU_F32 0          /* float in/out, float arithmetic: LaplCube<float>/LaplRect<float> */
+#define FDMGPU_F64 1          /* double in/out, double arithmetic: LaplCube<double>/LaplRect<double> */
+#define FDMGPU_F32_REFINE64 2 /* double in/out; FP32 transforms + tridiagonal, FP64 residual/update/norm */
+
+/* ---- periodic axes (tensor_flag::periodic per axis, src/tensor.h:12-32) ---*/
+#define FDMGPU_PERIODIC_X 1
+#define FDMGPU_PERIODIC_Y 2
+#define FDMGPU_PERIODIC_Z 4
+
+/* Replaces LaplCube<T,check,F>::LaplCube(dx,dy,dz,lx,ly,lz,nx,ny,nz)
+ * (src/lapl_cube.h:58-100) and its init_lm (src/lapl_cube.cpp:145-172).
+ * Transform lengths nx+1, ny+1, nz+1 (Dirichlet) or nx, ny, nz (periodic) must
+ * be powers of two.  periodic_mask: 0 (tensor_flags<>) or 7 (periodic^3), the
+ * reference's instantiations (src/lapl_cube.cpp:174-182). */
+int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double dz,
+                     double lx, double ly, double lz, int nx, int ny, int nz);
+
+/* Replaces LaplRect<T,check,F>::LaplRect(dx,dy,lx,ly,nx,ny) (src/lapl_rect.h:40-63).
+ * periodic_mask: 0 or FDMGPU_PERIODIC_Y (tensor_flags<periodic>, F::head = y).
+ * ny+1 (Dirichlet) / ny (periodic) must be a power of two; nx is arbitrary. */
+int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double lx,
+                     double ly, int nx, int ny);
+
+/* Replaces the public LaplRect members lm_y_scale, L_scale, U_scale
+ * (src/lapl_rect.h:32, 57-59): arrays of nx+1 doubles indexed 1..nx (entry 0
+ * unused, as in the reference); NULL keeps the default of all ones.  Returns
+ * FDMGPU_ERR_SINGULAR when a row would need pivoting (the GPU eliminates
+ * without row interchanges; the reference's gtsv pivots). */
+int fdmgpu_rect_set_scales(fdmgpu_plan plan, const double* lm_y_scale, const double* L_scale,
+                           const double* U_scale);
+
+/* FDMGPU_F32_REFINE64 only: stop when ||f - A u||_2 / ||f||_2 <= rel_tol
+ * (default 1e-10) or after max_iter FP32 inner solves (default 4).  No
+ * reference counterpart (the reference solves in one precision). */
+int fdmgpu_set_refine(fdmgpu_plan plan, double rel_tol, int max_iter);
+
+/* Replaces LaplCube::solve / LaplRect::solve(T* ans, T* rhs)
+ * (src/lapl_cube.cpp:9-142, src/lapl_rect.cpp:63-110): host pointers,
+ * synchronous, ans fully overwritten, rhs not modified, ans == rhs allowed. */
+int fdmgpu_solve_host(fdmgpu_plan plan, void* ans, const void* rhs);
+
+/* Same solve on device pointers, stream-ordered on `stream` (a hipStream_t,
+ * NULL = default stream); returns after enqueueing.  ans == rhs allowed.
+ * No reference counterpart (the reference has no device path). */
+int fdmgpu_solve_device(fdmgpu_plan plan, void* d_ans, const void* d_rhs, void* stream);
+
+/* `count` independent problems of the plan's shape laid out back to back
+ * (stride = field size) in one launch sequence.  No reference counterpart. */
+int fdmgpu_solve_batch_device(fdmgpu_plan plan, int count, void* d_ans, const void* d_rhs, void* stream);
+
+/* Statistics of the plan's last solve (synchronises the stream it ran on).
+ * iterations: FP32 inner solves (refine plans; 1 otherwise); rel_residual:
+ * ||f - A u||/||f|| of the last problem (refine plans; -1 otherwise). */
+typedef struct fdmgpu_stats {
+    int iterations;
+    double rel_residual;
+    double max_rel_residual; /* over a batch */
+} fdmgpu_stats;
+int fdmgpu_get_stats(fdmgpu_plan plan, fdmgpu_stats* stats);
+
+/* Device bytes the plan holds, and the extra bytes one solve of `count`
+ * problems needs (scratch); used for HBM-footprint admission. */
+int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resident,
+                          unsigned long long* per_solve);
+
+/* Per-kernel timing with HIP events recorded around every launch on the
+ * solve's stream (instrumentation for measurement; no reference counterpart --
+ * the reference times its phases with std::chrono, src/ns_cube.cpp:27-62).
+ * fdmgpu_set_timing(plan, 1) enables and resets the counters;
+ * fdmgpu_get_timing synchronises and returns the summed milliseconds and the
+ * launch count of one kernel class since the last reset. */
+#define FDMGPU_K_XY_FWD 0   /* plane DST kernel, forward (x and y of whole z-planes) */
+#define FDMGPU_K_XY_INV 1   /* plane DST kernel, inverse */
+#define FDMGPU_K_X_TILE 2   /* x-axis DST tiles (planes too large for LDS) */
+#define FDMGPU_K_Y_TILE 3   /* y-axis DST tiles */
+#define FDMGPU_K_ZCR 4      /* z cyclic reduction */
+#define FDMGPU_K_ROWCR 5    /* LaplRect row cyclic reduction */
+#define FDMGPU_K_RESIDUAL 6 /* FP64 residual + norm + convergence flag */
+#define FDMGPU_K_OTHER 7    /* flag fills, FP32->FP64 updates, copies */
+#define FDMGPU_NUM_KERNELS 8
+int fdmgpu_set_timing(fdmgpu_plan plan, int enable);
+int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launches);
+
+/* Replaces ~LaplCube / ~LaplRect. */
+int fdmgpu_destroy(fdmgpu_plan plan);
+
+const char* fdmgpu_strerror(int code);
+
+/* ABI version (major*10000 + minor*100 + patch). */
+int fdmgpu_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDMGPU_H */

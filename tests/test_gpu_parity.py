@@ -1,0 +1,258 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the
+reference's golden vectors.
+
+Tolerances (stated per test):
+  * FP64 plans: ||u_gpu - u_cpu||_2 / ||u_cpu||_2 <= 1e-12 (the north-star
+    contract "plain GPU FP64 must match the CPU path to <= 1e-12 relative");
+  * FP32 plans: relative L2 error vs the FP32 oracle <= 2e-5 (float rounding);
+  * FP32+refine plans: relative residual ||f - A u|| / ||f|| <= 1e-10 computed
+    in FP64 on the host, and error vs the FP64 oracle <= 1e-9.
+Every GPU step prints its case first (flushed) so a failing box names it.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"))
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def fdm():
+    import fdm_amd
+    return fdm_amd
+
+
+def rel2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def cube_setup(nx, ny, nz, seed=1, aniso=False):
+    if aniso:
+        dx, dy, dz = 0.11, 0.07, 0.13
+    else:
+        dx = dy = dz = 2 * np.pi / max(nx, ny, nz)
+    g = (dx, dy, dz, (nx + 1) * dx, (ny + 1) * dy, (nz + 1) * dz)
+    rhs = O.field(seed, (nz, ny, nx))
+    return g, rhs
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 3, 3), (7, 7, 7), (15, 15, 15), (31, 31, 31), (63, 63, 63),
+                                   (127, 127, 127), (31, 15, 7), (7, 31, 63), (63, 127, 31), (255, 15, 31)])
+def test_cube_f64_vs_oracle(fdm, shape):
+    nx, ny, nz = shape
+    say("cube f64", shape)
+    g, rhs = cube_setup(nx, ny, nz, seed=3, aniso=(nx != ny))
+    ref = O.lapl_cube(rhs, *g)
+    s = fdm.LaplCube(*g, nx, ny, nz, precision="f64")
+    ans = np.zeros_like(rhs)
+    s.solve(ans, rhs)
+    assert rel2(ans, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(7, 7, 7), (31, 31, 31), (127, 127, 127), (63, 31, 15), (255, 7, 7)])
+def test_cube_f32_vs_oracle(fdm, shape):
+    nx, ny, nz = shape
+    say("cube f32", shape)
+    g, rhs = cube_setup(nx, ny, nz, seed=4)
+    rhs32 = rhs.astype(np.float32)
+    ref = O.lapl_cube(rhs32, *g)
+    ref64 = O.lapl_cube(rhs32.astype(np.float64), *g)
+    s = fdm.LaplCube(*g, nx, ny, nz, precision="f32")
+    ans = np.zeros_like(rhs32)
+    s.solve(ans, rhs32)
+    # FP32 tolerance: both are ~1e-6 away from the exact FP64 answer
+    assert rel2(ans, ref64) <= 2e-5
+    assert rel2(ans, ref) <= 2e-5
+
+
+@pytest.mark.parametrize("n", [15, 31])
+@pytest.mark.parametrize("tag", ["d", "f"])
+def test_cube_vs_reference_golden(fdm, n, tag):
+    say("cube golden", n, tag)
+    rhs = GOLD[f"cube_{tag}_{n}_rhs"]
+    g = GOLD[f"cube_{tag}_{n}_geom"]
+    s = fdm.LaplCube(*g, n, n, n, precision="f64" if tag == "d" else "f32")
+    ans = np.zeros_like(rhs)
+    s.solve(ans, rhs)
+    assert rel2(ans, GOLD[f"cube_{tag}_{n}_ans"]) <= (1e-12 if tag == "d" else 2e-5)
+
+
+def test_cube_aniso_quirk_golden(fdm):
+    say("cube aniso golden")
+    rhs = GOLD["cube_aniso_d_rhs"]
+    g = GOLD["cube_aniso_d_geom"]
+    nz, ny, nx = rhs.shape
+    s = fdm.LaplCube(*g, nx, ny, nz, precision="f64")
+    ans = np.zeros_like(rhs)
+    s.solve(ans, rhs)
+    assert rel2(ans, GOLD["cube_aniso_d_ans"]) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [31, 127])
+def test_cube_manufactured_f64(fdm, n):
+    say("cube manufactured f64", n)
+    u, f, h, l = O.manufactured_cube(n, seed=1)
+    s = fdm.LaplCube(h, h, h, l, l, l, n, n, n, precision="f64")
+    ans = np.zeros_like(f)
+    s.solve(ans, f)
+    assert rel2(ans, u) <= 1e-12
+    assert O.rel_residual(f, ans, h, h, h) <= 1e-13
+
+
+def test_cube_inplace(fdm):
+    say("cube in place f64 31")
+    g, rhs = cube_setup(31, 31, 31, seed=5)
+    ref = O.lapl_cube(rhs, *g)
+    s = fdm.LaplCube(*g, 31, 31, 31)
+    buf = rhs.copy()
+    s.solve(buf, buf)
+    assert rel2(buf, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [31, 127])
+def test_cube_refine(fdm, n):
+    say("cube refine", n)
+    u, f, h, l = O.manufactured_cube(n, seed=2)
+    ref = O.lapl_cube(f, h, h, h, l, l, l)
+    s = fdm.LaplCube(h, h, h, l, l, l, n, n, n, precision="refine")
+    ans = np.zeros_like(f)
+    s.solve(ans, f)
+    st = s.stats()
+    res = O.rel_residual(f, ans, h, h, h)
+    print(f"   iterations={st.iterations} device_residual={st.rel_residual:.3e} host_residual={res:.3e} "
+          f"err_vs_fp64={rel2(ans, ref):.3e}", flush=True)
+    assert res <= 1e-10
+    assert st.rel_residual <= 1e-10
+    assert abs(st.rel_residual - res) <= 1e-12 + 0.05 * res
+    assert rel2(ans, ref) <= 1e-9
+
+
+def test_cube_device_batch_matches_single(fdm):
+    from fdm_amd import hip
+    say("cube batch of 3 on device, f64 and refine, 63^3")
+    n = 63
+    h, l = O.cube_geometry(n)
+    fs = np.stack([O.manufactured_cube(n, seed=s)[1] for s in (11, 12, 13)])
+    for prec in ("f64", "refine"):
+        s = fdm.LaplCube(h, h, h, l, l, l, n, n, n, precision=prec)
+        rhs = hip.DeviceArray.from_host(fs)
+        out = hip.DeviceArray(fs.shape, np.float64)
+        s.solve_batch(out, rhs)
+        hip.synchronize()
+        batch = out.download()
+        for i in range(3):
+            one_r = hip.DeviceArray.from_host(fs[i])
+            one = hip.DeviceArray(fs[i].shape, np.float64)
+            s.solve(one, one_r)
+            hip.synchronize()
+            # a problem's result does not depend on its batch neighbours: bitwise
+            assert np.array_equal(one.download(), batch[i])
+        ref = O.lapl_cube(fs[1], h, h, h, l, l, l)
+        assert rel2(batch[1], ref) <= (1e-12 if prec == "f64" else 1e-9)
+
+
+def test_torch_interop_subprocess():
+    # torch brings its own HIP runtime; with torch initialised first, libfdmgpu.so binds
+    # to it and accepts torch tensors.  Run in a fresh process.
+    import subprocess
+    say("torch interop (subprocess)")
+    code = r"""
+import torch, numpy as np
+assert torch.cuda.is_available()
+torch.zeros(1, device='cuda')
+import fdm_amd, oracle as O
+n = 31
+u, f, h, l = O.manufactured_cube(n, 1)
+s = fdm_amd.LaplCube(h, h, h, l, l, l, n, n, n, precision='f64')
+rhs = torch.from_numpy(f).cuda(); out = torch.zeros_like(rhs)
+s.solve(out, rhs); torch.cuda.synchronize()
+err = np.linalg.norm(out.cpu().numpy() - u) / np.linalg.norm(u)
+print('torch-interop err', err); assert err < 1e-12
+"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=180)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0
+
+
+@pytest.mark.parametrize("shape", [(31, 31), (100, 63), (7, 127), (511, 255)])
+def test_rect_f64_vs_oracle(fdm, shape):
+    nx, ny = shape
+    say("rect f64", shape)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    rhs = O.field(8, (ny, nx))
+    ref = O.lapl_rect(rhs, dx, dy, 1 + dx, 1 + dy)
+    s = fdm.LaplRect(dx, dy, 1 + dx, 1 + dy, nx, ny)
+    ans = np.zeros_like(rhs)
+    s.solve(ans, rhs)
+    assert rel2(ans, ref) <= 1e-12
+
+
+def test_rect_scales_vs_oracle(fdm):
+    say("rect scales f64 (cylindrical-like)")
+    nx, ny = 40, 31
+    dr, dy = 0.05, 0.1
+    r = 1.0 + dr * np.arange(0, nx + 1)
+    lm = np.r_[0, 1.0 / r[1:] ** 2]
+    Ls = np.r_[0, (r[1:] - 0.5 * dr) / r[1:]]
+    Us = np.r_[0, (r[1:] + 0.5 * dr) / r[1:]]
+    rhs = O.field(9, (ny, nx))
+    ref = O.lapl_rect(rhs, dr, dy, (nx + 1) * dr, (ny + 1) * dy, lm_y_scale=lm, L_scale=Ls, U_scale=Us)
+    for prec, tol in (("f64", 1e-12), ("refine", 1e-9)):
+        s = fdm.LaplRect(dr, dy, (nx + 1) * dr, (ny + 1) * dy, nx, ny, precision=prec)
+        s.set_scales(lm, Ls, Us)
+        ans = np.zeros_like(rhs)
+        s.solve(ans, rhs)
+        assert rel2(ans, ref) <= tol
+
+
+def test_rect_biquadratic_exact(fdm):
+    # ut/ut_lapl_rect.cpp:300-374: exact for a biquadratic, < 1e-14 (double) / 1e-5 (float)
+    say("rect biquadratic")
+    n = 31
+    h = 1.0 / n
+    K, J = np.meshgrid(np.arange(1, n + 1), np.arange(1, n + 1), indexing="ij")
+
+    def a0(k, j):
+        x, y = h * j - h / 2, h * k - h / 2
+        return x * (x - 1) * y * (y - 1)
+
+    x, y = h * J - h / 2, h * K - h / 2
+    rhs = 2 * y * (y - 1) + 2 * x * (x - 1)
+    rhs[0] -= a0(K[0] - 1, J[0]) / h / h
+    rhs[:, 0] -= a0(K[:, 0], J[:, 0] - 1) / h / h
+    rhs[:, -1] -= a0(K[:, -1], J[:, -1] + 1) / h / h
+    rhs[-1] -= a0(K[-1] + 1, J[-1]) / h / h
+    exact = a0(K, J)
+    for prec, dt, tol in (("f64", np.float64, 1e-14), ("f32", np.float32, 1e-5)):
+        s = fdm.LaplRect(h, h, 1 + h, 1 + h, n, n, precision=prec)
+        ans = np.zeros((n, n), dt)
+        s.solve(ans, rhs.astype(dt))
+        assert np.max(np.abs(ans - exact)) / np.max(np.abs(exact)) < tol
+
+
+def test_errors(fdm):
+    say("error paths")
+    with pytest.raises(fdm.FdmGpuError) as e:
+        fdm.LaplCube(0.1, 0.1, 0.1, 1, 1, 1, 6, 7, 7)       # nx+1 = 7 is not a power of two
+    assert e.value.code == fdm.ERR_INVALID_SIZE
+    with pytest.raises(fdm.FdmGpuError) as e:
+        fdm.LaplRect(0.1, 0.1, 1, 1, 10, 10)                  # ny+1 = 11
+    assert e.value.code == fdm.ERR_INVALID_SIZE
+    s = fdm.LaplRect(0.1, 0.1, 1.1, 3.2, 10, 31)
+    with pytest.raises(fdm.FdmGpuError) as e:                 # would need pivoting
+        s.set_scales(np.zeros(11), np.full(11, 5.0), np.full(11, 5.0))
+    assert e.value.code == fdm.ERR_SINGULAR
