@@ -170,7 +170,22 @@ def main():
     ap.add_argument("--n", type=int, default=127)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--mode", choices=("both", "refine", "f64"), default="both",
+                    help="which GPU legs to run (profiling runs use one)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run device-to-device copies of this many MiB (calibrates the HBM "
+                         "FETCH_SIZE/WRITE_SIZE counters against a copy of known size) and exit")
     args = ap.parse_args()
+    if args.calib_copy_mib:
+        from fdm_amd import hip
+        nb = args.calib_copy_mib << 20
+        a = hip.DeviceArray((nb // 8,), np.float64)
+        b = hip.DeviceArray((nb // 8,), np.float64)
+        for _ in range(3):
+            hip.check(hip.rt().hipMemcpy(b.ptr, a.ptr, nb, hip.D2D), "copy")
+        hip.synchronize()
+        print(json.dumps({"calib_copy_bytes": nb, "copies": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the batched solve shards by whole "
                                    "problems across GPUs (see DESIGN.md); run with --gpus 1"}))
@@ -187,6 +202,12 @@ def main():
     F = np.stack(fs)
     del fs
 
+    if args.mode != "both":
+        r = run_gpu(args.mode, n, B, args.steps, args.warmup, F)
+        print(json.dumps({"mode": args.mode, "solves_per_s": B * args.steps / (r["wall_ms"] / 1e3),
+                          "iterations": r["iterations"],
+                          "kernel_ms_per_step": {k: v[0] / args.steps for k, v in r["timing"].items()}}))
+        return
     res_r = run_gpu("refine", n, B, args.steps, args.warmup, F)
     res_d = run_gpu("f64", n, B, args.steps, args.warmup, F)
 

@@ -130,18 +130,71 @@ struct DstShape {
     static constexpr int N2 = N / N1;            // N2 >= N1
     static constexpr int M = N / 2;
     static constexpr int KC = M / N1;            // post-processing outputs per lane (>= 1)
+    static constexpr int LOGKC = ilog2(KC);
     // LDS padding: a pencil-to-pencil stride congruent to N1 mod 32 banks spreads
     // the N1 lanes of neighbouring jobs over distinct banks.
     static constexpr int PAD = N1;
+
+    // Slot of the step-1 result Y[j1][k2] (XOR swizzle: conflict-free for the
+    // step-1 stores, contiguous in j1, and the step-2 loads, contiguous in k2).
+    __device__ static constexpr int posY(int j1, int k2) { return N1 * k2 + (j1 ^ (k2 & (N1 - 1))); }
+    // Slot of the FFT output Z_k (swizzled so step 3's per-lane k ranges hit
+    // distinct banks).
+    __device__ static constexpr int posZ(int k) {
+        return KC > 1 ? ((k & ~(KC - 1)) | ((k ^ (k >> LOGKC)) & (KC - 1))) : k;
+    }
+    // Slot that receives the transform output X_j.  Pencils are independent, so
+    // the outputs may sit in any fixed permutation between passes.  Step 3's
+    // lane t writes j = N2*t + c (c fixed per store), so XOR-ing the low bits of
+    // j with t makes those stores hit distinct banks, while runs of consecutive
+    // j stay conflict-free for the natural-order reads.  sigma is an
+    // involution and sigma(0) = 0.
+    static constexpr int LOGN2 = ilog2(N2);
+    __device__ static constexpr int sigma(int j) { return j ^ ((j >> LOGN2) & (N1 - 1)); }
 };
 
-// Per-axis constant tables in LDS: twr[m], twi[m] = cos, -sin(2 pi m / N),
-// sn[j] = sin(pi j / N), m, j in [0, N).
-template <typename T>
-struct DstTables {
-    const T* twr;
-    const T* twi;
-    const T* sn;
+// Per-lane constants of the DST (the lane's index t inside its job is fixed for
+// the whole kernel): twiddles w_N^{t k2}, k2 < N2, and sines sin(pi (t+N1 j2)/N).
+template <typename T, int LOGN>
+struct DstLane {
+    using S = DstShape<LOGN>;
+    T twr[S::N2], twi[S::N2], sn[S::N2];
+    // tables (global memory): cos(2 pi m/N), -sin(2 pi m/N), sin(pi m/N), m < N
+    __device__ __forceinline__ void init(const T* __restrict__ tables, const int t) {
+        constexpr int N = S::N, N1 = S::N1, N2 = S::N2;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            twr[k2] = tables[t * k2];
+            twi[k2] = tables[N + t * k2];
+            sn[k2] = tables[2 * N + t + N1 * k2];
+        }
+    }
+};
+
+// LDS layouts of a set of pencils.  Measured on gfx950: b32 accesses and b64
+// reads see 32 banks per half-wavefront (b64 reads in 8-byte banks), b64
+// writes 16 eight-byte banks per quarter-wavefront.  A job's N1 lanes touch N1
+// slots that are distinct mod N1, so a row pitch congruent to N1 mod 32 makes
+// row-pencil (x) accesses conflict-free.  Column-pencil (y) accesses are then
+// spread by XOR-ing the column index with a function of the row, f(row).
+template <int N1>
+struct RowSwizzle {
+    static constexpr int QJ = 32 / N1;                            // jobs per half-wavefront
+    static constexpr int LS = ilog2(QJ);
+    static constexpr int FM = (N1 * N1 > 32) ? (N1 * N1) / 32 - 1 : 0;
+    static constexpr int FMAX = FM << LS;                         // largest f value
+    __device__ static constexpr int f(int row) { return ((row >> LS) & FM) << LS; }
+};
+
+// Plane of rows with pitch P; element (row, col) at row*P + (col ^ f(row)).
+// Pencils are rows (x pass) or columns (y pass).
+template <int P, int N1, bool ALONG_X>
+struct PlaneLayout {
+    using Sw = RowSwizzle<N1>;
+    __device__ static constexpr int at(int row, int col) { return row * P + (col ^ Sw::f(row)); }
+    __device__ static constexpr int addr(int pencil, int slot) {
+        return ALONG_X ? at(pencil, slot) : at(slot, pencil);
+    }
 };
 
 // Inclusive scan across `width` consecutive lanes (width a power of two <= 64).
@@ -155,77 +208,78 @@ __device__ __forceinline__ T lane_scan_incl(T v) {
     return v;
 }
 
-// DST-I of 2*JOBS pencils in LDS: pencil p, slot j at lds[p*SP + j*SJ];
-// job g transforms pencils g and g + JOBS.  Slots 1..N-1 hold x_1..x_{N-1}
-// on entry and scale*X_1..scale*X_{N-1} on exit; slot 0 is ignored on entry
-// and 0 on exit.  Every thread of the block must call this (block barriers).
-template <typename T, int LOGN, int JOBS>
-__device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const int SP, const int SJ,
-                                          const DstTables<T> tb, const T scale, const int tid,
-                                          const int nthreads) {
+// Orders the LDS accesses of one wavefront across lanes.  A job's lanes lie in
+// one wavefront, whose LDS instructions execute in issue order, so only the
+// compiler's code motion has to be fenced.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// DST-I of 2*JOBS pencils in LDS: pencil p, slot j at lds[Lay::addr(p, j)];
+// job g transforms pencils g and g + JOBS.  On entry slots 1..N-1 hold
+// x_1..x_{N-1} in natural order and slot 0 holds a finite value; on exit slot
+// sigma(j) holds scale*X_j (j = 1..N-1) and slot 0 holds 0.  All NT threads of
+// the block call this; no block barrier inside (jobs never straddle a wavefront).
+template <typename T, int LOGN, int JOBS, typename Lay, int NT>
+__device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, LOGN>& L, const T scale,
+                                          const int tid) {
     using S = DstShape<LOGN>;
     constexpr int N = S::N, N1 = S::N1, N2 = S::N2, KC = S::KC;
     constexpr int WORK = JOBS * N1;
-    for (int base = 0; base < WORK; base += nthreads) {
+    static_assert(NT % N1 == 0 && 64 % N1 == 0, "jobs must not straddle wavefronts");
+    const int t = tid & (N1 - 1);
+#pragma unroll 1
+    for (int base = 0; base < WORK; base += NT) {
         const int w = base + tid;
-        const bool active = w < WORK;
-        const int g = active ? w / N1 : 0;
-        const int t = w & (N1 - 1);
-        T* pa = lds + g * SP;
-        T* pb = lds + (g + JOBS) * SP;
+        if (w < WORK) {
+            const int g = w / N1;
+            const int gb = g + JOBS;
+#define PA(j) lds[Lay::addr(g, (j))]
+#define PB(j) lds[Lay::addr(gb, (j))]
 
-        // ---- step 1: pre-processing + N2-point DFTs over j2 (j = t + N1*j2)
-        T yr[N2], yi[N2];
-        if (active) {
+            // ---- step 1: pre-processing + N2-point DFTs over j2 (j = t + N1*j2)
+            T yr[N2], yi[N2];
 #pragma unroll
             for (int j2 = 0; j2 < N2; ++j2) {
                 const int j = t + N1 * j2;
-                if (j == 0) {
-                    yr[j2] = T(0);
-                    yi[j2] = T(0);
-                } else {
-                    const T a0 = pa[j * SJ], a1 = pa[(N - j) * SJ];
-                    const T b0 = pb[j * SJ], b1 = pb[(N - j) * SJ];
-                    const T s = tb.sn[j];
-                    yr[j2] = s * (a0 + a1) + T(0.5) * (a0 - a1);
-                    yi[j2] = s * (b0 + b1) + T(0.5) * (b0 - b1);
-                }
+                const int jn = j2 == 0 ? ((N - t) & (N - 1)) : N - j;   // slot 0 for j = 0
+                const T a0 = PA(j), a1 = PA(jn);
+                const T b0 = PB(j), b1 = PB(jn);
+                const T s = L.sn[j2];
+                yr[j2] = s * (a0 + a1) + T(0.5) * (a0 - a1);
+                yi[j2] = s * (b0 + b1) + T(0.5) * (b0 - b1);
             }
             dft<N2>(yr, yi);
 #pragma unroll
             for (int k2 = 1; k2 < N2; ++k2) {
-                const int m = t * k2;   // < N
-                const T c = tb.twr[m], s = tb.twi[m];
+                const T c = L.twr[k2], s = L.twi[k2];
                 const T r = yr[k2] * c - yi[k2] * s;
                 const T i = yr[k2] * s + yi[k2] * c;
                 yr[k2] = r;
                 yi[k2] = i;
             }
-        }
-        __syncthreads();
-        if (active) {
+            wave_lds_sync();
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
-                const int pos = t + N1 * k2;
-                pa[pos * SJ] = yr[k2];
-                pb[pos * SJ] = yi[k2];
+                const int pos = S::posY(t, k2);
+                PA(pos) = yr[k2];
+                PB(pos) = yi[k2];
             }
-        }
-        __syncthreads();
+            wave_lds_sync();
 
-        // ---- step 2: N1-point DFTs over j1 for k2 = t + N1*r
-        constexpr int RR = N2 / N1;
-        T zr[N2], zi[N2];
-        if (active) {
+            // ---- step 2: N1-point DFTs over j1 for k2 = t + N1*r
+            constexpr int RR = N2 / N1;
+            T zr[N2], zi[N2];
 #pragma unroll
             for (int r = 0; r < RR; ++r) {
                 const int k2 = t + N1 * r;
                 T vr[N1], vi[N1];
 #pragma unroll
                 for (int j1 = 0; j1 < N1; ++j1) {
-                    const int pos = j1 + N1 * k2;
-                    vr[j1] = pa[pos * SJ];
-                    vi[j1] = pb[pos * SJ];
+                    const int pos = S::posY(j1, k2);
+                    vr[j1] = PA(pos);
+                    vi[j1] = PB(pos);
                 }
                 dft<N1>(vr, vi);
 #pragma unroll
@@ -234,32 +288,27 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const int SP, con
                     zi[r * N1 + k1] = vi[k1];
                 }
             }
-        }
-        __syncthreads();
-        if (active) {
+            wave_lds_sync();
 #pragma unroll
             for (int r = 0; r < RR; ++r) {
-                const int k2 = t + N1 * r;
 #pragma unroll
                 for (int k1 = 0; k1 < N1; ++k1) {
-                    const int k = k2 + N2 * k1;
-                    pa[k * SJ] = zr[r * N1 + k1];
-                    pb[k * SJ] = zi[r * N1 + k1];
+                    const int pos = S::posZ(t + N1 * r + N2 * k1);
+                    PA(pos) = zr[r * N1 + k1];
+                    PB(pos) = zi[r * N1 + k1];
                 }
             }
-        }
-        __syncthreads();
+            wave_lds_sync();
 
-        // ---- step 3: even outputs directly, odd outputs by prefix sum
-        T ae[KC], ao[KC], be[KC], bo[KC];
-        if (active) {
+            // ---- step 3: even outputs directly, odd outputs by prefix sum
+            T ae[KC], ao[KC], be[KC], bo[KC];
             T runA = T(0), runB = T(0);
 #pragma unroll
             for (int i = 0; i < KC; ++i) {
                 const int k = t * KC + i;
-                const int kn = (N - k) & (N - 1);
-                const T zkr = pa[k * SJ], zki = pb[k * SJ];
-                const T znr = pa[kn * SJ], zni = pb[kn * SJ];
+                const int pk = S::posZ(k), pn = S::posZ((N - k) & (N - 1));
+                const T zkr = PA(pk), zki = PB(pk);
+                const T znr = PA(pn), zni = PB(pn);
                 ae[i] = T(0.5) * (zni - zki);
                 be[i] = T(0.5) * (zkr - znr);
                 T dA = T(0.5) * (zkr + znr);
@@ -274,33 +323,30 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const int SP, con
                 bo[i] = runB;
             }
             if constexpr (N1 > 1) {
-                const T inclA = lane_scan_incl<N1>(runA);
-                const T inclB = lane_scan_incl<N1>(runB);
-                const T offA = inclA - runA, offB = inclB - runB;
+                const T offA = lane_scan_incl<N1>(runA) - runA;
+                const T offB = lane_scan_incl<N1>(runB) - runB;
 #pragma unroll
                 for (int i = 0; i < KC; ++i) {
                     ao[i] += offA;
                     bo[i] += offB;
                 }
             }
-        }
-        __syncthreads();
-        if (active) {
+            wave_lds_sync();
 #pragma unroll
             for (int i = 0; i < KC; ++i) {
                 const int k = t * KC + i;
-                if (k == 0) {
-                    pa[0] = T(0);
-                    pb[0] = T(0);
-                } else {
-                    pa[(2 * k) * SJ] = ae[i] * scale;
-                    pb[(2 * k) * SJ] = be[i] * scale;
-                }
-                pa[(2 * k + 1) * SJ] = ao[i] * scale;
-                pb[(2 * k + 1) * SJ] = bo[i] * scale;
+                const int pe = S::sigma(2 * k), po = S::sigma(2 * k + 1);
+                const T e_a = k == 0 ? T(0) : ae[i] * scale;
+                const T e_b = k == 0 ? T(0) : be[i] * scale;
+                PA(pe) = e_a;
+                PB(pe) = e_b;
+                PA(po) = ao[i] * scale;
+                PB(po) = bo[i] * scale;
             }
+            wave_lds_sync();
+#undef PA
+#undef PB
         }
-        __syncthreads();
     }
 }
 

@@ -36,7 +36,7 @@ struct PlaneCfg {
     static constexpr int NT = WORK < 64 ? 64 : WORK;
     template <typename T>
     static constexpr size_t lds_bytes() {
-        return sizeof(T) * ((size_t)N * P + 3 * N);
+        return sizeof(T) * (size_t)N * P;
     }
 };
 
@@ -45,116 +45,129 @@ __global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_dst_kernel(
     const Tin* __restrict__ in, Tout* __restrict__ out, const long bs_in, const long bs_out, const T sx,
     const T sy, const T* __restrict__ tables, const int* __restrict__ done) {
     using C = PlaneCfg<LOGN>;
+    using S = DstShape<LOGN>;
+    using LX = PlaneLayout<C::P, S::N1, true>;
+    using LY = PlaneLayout<C::P, S::N1, false>;
     if (done && done[blockIdx.y]) return;
-    constexpr int N = C::N, n = N - 1, P = C::P, NT = C::NT;
+    constexpr int N = C::N, n = N - 1, NT = C::NT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
-    T* tab = lds + N * P;
     const int tid = threadIdx.x;
     const long plane_off = (long)blockIdx.x * n * n;
     const Tin* src = in + blockIdx.y * bs_in + plane_off;
     Tout* dst = out + blockIdx.y * bs_out + plane_off;
 
-    for (int i = tid; i < 3 * N; i += NT) tab[i] = tables[i];
+    DstLane<T, LOGN> L;
+    L.init(tables, tid & (S::N1 - 1));
     for (int i = tid; i < N; i += NT) {
-        lds[i] = T(0);       // dummy row 0
-        lds[i * P] = T(0);   // dummy column 0
+        lds[LX::at(0, i)] = T(0);   // dummy row 0
+        lds[LX::at(i, 0)] = T(0);   // dummy column 0
     }
     for (int idx = tid; idx < n * n; idx += NT) {
         const int y = idx / n, x = idx - y * n;
-        lds[(y + 1) * P + x + 1] = (T)src[idx];
+        lds[LX::at(y + 1, x + 1)] = (T)src[idx];
     }
     __syncthreads();
-    const DstTables<T> tb{tab, tab + N, tab + 2 * N};
-    dst_pairs<T, LOGN, N / 2>(lds, P, 1, tb, sx, tid, NT);   // pencils = rows  (x)
-    dst_pairs<T, LOGN, N / 2>(lds, 1, P, tb, sy, tid, NT);   // pencils = columns (y)
+    dst_pairs<T, LOGN, N / 2, LX, NT>(lds, L, sx, tid);   // pencils = rows    (x)
+    __syncthreads();
+    dst_pairs<T, LOGN, N / 2, LY, NT>(lds, L, sy, tid);   // pencils = columns (y)
+    __syncthreads();
     for (int idx = tid; idx < n * n; idx += NT) {
         const int y = idx / n, x = idx - y * n;
-        if constexpr (ACCUM) dst[idx] += (Tout)lds[(y + 1) * P + x + 1];
-        else dst[idx] = (Tout)lds[(y + 1) * P + x + 1];
+        const T v = lds[LX::at(S::sigma(y + 1), S::sigma(x + 1))];
+        if constexpr (ACCUM) dst[idx] += (Tout)v;
+        else dst[idx] = (Tout)v;
     }
 }
 
 // ---------------------------------------------------------------------------
 // axis-tile kernels (planes that do not fit LDS, or non-square planes)
 // ---------------------------------------------------------------------------
-template <int LOGN>
+template <int LOGN, typename T>
 struct TileCfg {
     using S = DstShape<LOGN>;
     static constexpr int N = S::N;
-    static constexpr int TILE = 32;                 // pencils per block (even)
-    static constexpr int P = N + S::PAD;            // x tiles: pencil stride
-    static constexpr int SJ = TILE + 64 / S::N1;    // y tiles: slot stride
+    // pencils per block: 32, or 16 when a 32-wide y tile would not fit LDS
+    static constexpr bool BIG = (size_t)N * (32 + S::N1) * sizeof(T) > 160 * 1024;
+    static constexpr int TILE = BIG ? 16 : 32;
+    static constexpr int SWN1 = (TILE / 2 > RowSwizzle<S::N1>::FMAX) ? S::N1 : 1;   // swizzle only if it fits
+    static constexpr int P = N + S::PAD;            // x tiles: pencil pitch
+    static constexpr int SJ = TILE + S::N1;         // y tiles: slot pitch
     static constexpr int WORK = (TILE / 2) * S::N1;
     static constexpr int NT = WORK < 64 ? 64 : WORK;
-    template <typename T>
-    static constexpr size_t lds_x() { return sizeof(T) * ((size_t)TILE * P + 3 * N); }
-    template <typename T>
-    static constexpr size_t lds_y() { return sizeof(T) * ((size_t)N * SJ + 3 * N); }
+    static constexpr size_t lds_x() { return sizeof(T) * (size_t)TILE * P; }
+    static constexpr size_t lds_y() { return sizeof(T) * (size_t)N * SJ; }
 };
 
 // rows = nrows pencils of length n = N-1, contiguous; batch via blockIdx.y.
 template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM>
-__global__ __launch_bounds__(TileCfg<LOGN>::NT) void xaxis_dst_kernel(
+__global__ __launch_bounds__((TileCfg<LOGN, T>::NT)) void xaxis_dst_kernel(
     const Tin* __restrict__ in, Tout* __restrict__ out, const long nrows, const long bs_in, const long bs_out,
     const T scale, const T* __restrict__ tables, const int* __restrict__ done) {
-    using C = TileCfg<LOGN>;
+    using C = TileCfg<LOGN, T>;
+    using S = DstShape<LOGN>;
+    using LX = PlaneLayout<C::P, S::N1, true>;
     if (done && done[blockIdx.y]) return;
-    constexpr int N = C::N, n = N - 1, P = C::P, NT = C::NT, TILE = C::TILE;
+    constexpr int N = C::N, n = N - 1, NT = C::NT, TILE = C::TILE;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
-    T* tab = lds + TILE * P;
     const int tid = threadIdx.x;
     const long r0 = (long)blockIdx.x * TILE;
     const Tin* src = in + blockIdx.y * bs_in + r0 * n;
     Tout* dst = out + blockIdx.y * bs_out + r0 * n;
     const int rows_here = (int)((nrows - r0) < TILE ? (nrows - r0) : TILE);
 
-    for (int i = tid; i < 3 * N; i += NT) tab[i] = tables[i];
+    DstLane<T, LOGN> L;
+    L.init(tables, tid & (S::N1 - 1));
+    for (int p = tid; p < TILE; p += NT) lds[LX::at(p, 0)] = T(0);
     for (int idx = tid; idx < TILE * n; idx += NT) {
         const int p = idx / n, x = idx - p * n;
-        lds[p * P + x + 1] = p < rows_here ? (T)src[idx] : T(0);
+        lds[LX::at(p, x + 1)] = p < rows_here ? (T)src[idx] : T(0);
     }
     __syncthreads();
-    const DstTables<T> tb{tab, tab + N, tab + 2 * N};
-    dst_pairs<T, LOGN, TILE / 2>(lds, P, 1, tb, scale, tid, NT);
+    dst_pairs<T, LOGN, TILE / 2, LX, NT>(lds, L, scale, tid);
+    __syncthreads();
     for (int idx = tid; idx < rows_here * n; idx += NT) {
         const int p = idx / n, x = idx - p * n;
-        if constexpr (ACCUM) dst[idx] += (Tout)lds[p * P + x + 1];
-        else dst[idx] = (Tout)lds[p * P + x + 1];
+        const T v = lds[LX::at(p, S::sigma(x + 1))];
+        if constexpr (ACCUM) dst[idx] += (Tout)v;
+        else dst[idx] = (Tout)v;
     }
 }
 
 // Columns of plane blockIdx.y (nplanes planes of [n][nx]); batch via blockIdx.z.
 template <typename Tin, typename Tout, typename T, int LOGN>
-__global__ __launch_bounds__(TileCfg<LOGN>::NT) void yaxis_dst_kernel(
+__global__ __launch_bounds__((TileCfg<LOGN, T>::NT)) void yaxis_dst_kernel(
     const Tin* __restrict__ in, Tout* __restrict__ out, const int nx, const long bs_in, const long bs_out,
     const T scale, const T* __restrict__ tables, const int* __restrict__ done) {
-    using C = TileCfg<LOGN>;
-    constexpr int N = C::N, n = N - 1, SJ = C::SJ, NT = C::NT, TILE = C::TILE;
+    using C = TileCfg<LOGN, T>;
+    using S = DstShape<LOGN>;
+    using LY = PlaneLayout<C::SJ, C::SWN1, false>;   // rows = slots, columns = pencils
+    constexpr int N = C::N, n = N - 1, NT = C::NT, TILE = C::TILE;
     if (done && done[blockIdx.z]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
-    T* tab = lds + N * SJ;
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TILE;
     const long poff = (long)blockIdx.y * n * nx;
     const Tin* src = in + blockIdx.z * bs_in + poff;
     Tout* dst = out + blockIdx.z * bs_out + poff;
 
-    for (int i = tid; i < 3 * N; i += NT) tab[i] = tables[i];
+    DstLane<T, LOGN> L;
+    L.init(tables, tid & (S::N1 - 1));
+    for (int p = tid; p < TILE; p += NT) lds[LY::addr(p, 0)] = T(0);   // slot 0 of every pencil
     for (int idx = tid; idx < n * TILE; idx += NT) {
         const int y = idx / TILE, p = idx - y * TILE;
         const int x = x0 + p;
-        lds[(y + 1) * SJ + p] = x < nx ? (T)src[(long)y * nx + x] : T(0);
+        lds[LY::addr(p, y + 1)] = x < nx ? (T)src[(long)y * nx + x] : T(0);
     }
     __syncthreads();
-    const DstTables<T> tb{tab, tab + N, tab + 2 * N};
-    dst_pairs<T, LOGN, TILE / 2>(lds, 1, SJ, tb, scale, tid, NT);
+    dst_pairs<T, LOGN, TILE / 2, LY, NT>(lds, L, scale, tid);
+    __syncthreads();
     for (int idx = tid; idx < n * TILE; idx += NT) {
         const int y = idx / TILE, p = idx - y * TILE;
         const int x = x0 + p;
-        if (x < nx) dst[(long)y * nx + x] = (Tout)lds[(y + 1) * SJ + p];
+        if (x < nx) dst[(long)y * nx + x] = (Tout)lds[LY::addr(p, S::sigma(y + 1))];
     }
 }
 
@@ -383,14 +396,14 @@ template <typename T>
 bool plane_fits(int logn) {
     if (logn < 1 || logn > 7) return false;
     size_t N = (size_t)1 << logn, N1 = (size_t)1 << (logn / 2);
-    return sizeof(T) * (N * (N + N1) + 3 * N) <= 160 * 1024;
+    return sizeof(T) * (N * (N + N1)) <= 160 * 1024;
 }
 
 template <typename Tin, typename Tout, typename T, int LOGN>
 static hipError_t xaxis_t(const AxisArgs& a) {
-    using C = TileCfg<LOGN>;
+    using C = TileCfg<LOGN, T>;
     auto k = a.accum ? xaxis_dst_kernel<Tin, Tout, T, LOGN, true> : xaxis_dst_kernel<Tin, Tout, T, LOGN, false>;
-    const size_t lds = C::template lds_x<T>();
+    const size_t lds = C::lds_x();
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     const long tiles = (a.nrows + C::TILE - 1) / C::TILE;
@@ -401,9 +414,9 @@ static hipError_t xaxis_t(const AxisArgs& a) {
 
 template <typename Tin, typename Tout, typename T, int LOGN>
 static hipError_t yaxis_t(const AxisArgs& a) {
-    using C = TileCfg<LOGN>;
+    using C = TileCfg<LOGN, T>;
     auto k = yaxis_dst_kernel<Tin, Tout, T, LOGN>;
-    const size_t lds = C::template lds_y<T>();
+    const size_t lds = C::lds_y();
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     const int tiles = (a.nx + C::TILE - 1) / C::TILE;

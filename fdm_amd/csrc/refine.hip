@@ -36,7 +36,8 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* sh) {
 }
 
 // r = f - A u (FP64), r32 = (float) r; partial[p][blk] = (sum r^2, sum f^2).
-// Cube: one block per 256 consecutive points of [nz][ny][nx].
+// Grid-stride over the points of problem blockIdx.y with a capped grid, so a
+// launch for already-converged problems costs a few microseconds.
 template <bool THREE, bool SCALED>
 __global__ __launch_bounds__(kResBlock) void residual_kernel(
     const double* __restrict__ f, const double* __restrict__ u, float* __restrict__ r32,
@@ -47,11 +48,12 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
     const int p = blockIdx.y;
     if (done && done[p]) return;
     const long elems = (long)nx * ny * nz;
-    const long idx = (long)blockIdx.x * kResBlock + threadIdx.x;
     const double* fp = f + p * bs;
     const double* up = u + p * bs;
+    float* rp = r32 + p * bs;
+    const long pl = (long)nx * ny;
     double rr = 0.0, ff = 0.0;
-    if (idx < elems) {
+    for (long idx = (long)blockIdx.x * kResBlock + threadIdx.x; idx < elems; idx += (long)gridDim.x * kResBlock) {
         const int x = (int)(idx % nx);
         const long t = idx / nx;
         const int y = (int)(t % ny);
@@ -66,15 +68,14 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
             au = (xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy;
         }
         if constexpr (THREE) {
-            const long pl = (long)nx * ny;
             const double zm = z > 0 ? up[idx - pl] : 0.0, zp = z < nz - 1 ? up[idx + pl] : 0.0;
             au += (zm - 2.0 * uc + zp) * iz;
         }
         const double fv = fp[idx];
         const double r = fv - au;
-        r32[p * bs + idx] = (float)r;
-        rr = r * r;
-        ff = fv * fv;
+        rp[idx] = (float)r;
+        rr += r * r;
+        ff += fv * fv;
     }
     block_sum2<double>(rr, ff, sh);
     if (threadIdx.x == 0) {
@@ -110,9 +111,10 @@ __global__ void fill_int_kernel(int* __restrict__ a, const int n, const int v) {
     if (i < n) a[i] = v;
 }
 
+constexpr long kResMaxBlocks = 1024;   // per problem
+
 hipError_t launch_residual(const ResidualArgs& a) {
-    const long elems = (long)a.nx * a.ny * a.nz;
-    const unsigned nblk = (unsigned)((elems + kResBlock - 1) / kResBlock);
+    const unsigned nblk = (unsigned)residual_partials(a.nx, a.ny, a.nz);
     dim3 grid(nblk, a.batch);
     if (a.nz > 1 || a.three) {
         hipLaunchKernelGGL((residual_kernel<true, false>), grid, dim3(kResBlock), 0, a.stream, a.f, a.u, a.r32,
@@ -133,7 +135,8 @@ hipError_t launch_residual(const ResidualArgs& a) {
 
 long residual_partials(int nx, int ny, int nz) {
     const long elems = (long)nx * ny * nz;
-    return (elems + kResBlock - 1) / kResBlock;
+    const long b = (elems + kResBlock - 1) / kResBlock;
+    return b < kResMaxBlocks ? b : kResMaxBlocks;
 }
 
 hipError_t launch_fill_int(int* a, int n, int v, hipStream_t s) {
