@@ -1,0 +1,132 @@
+// C++ drop-in check: the reference's unit-test scenarios (ut/ut_lapl_cube.cpp,
+// ut/ut_lapl_rect.cpp) written against include/fdm/*.h, i.e. the way an fdm
+// caller uses LaplCube / LaplRect, running on the GPU through the C ABI.
+//
+// Built by __graft_entry__.build() (g++ against libfdmgpu.so); run by
+// tests/test_cpp_dropin.py on a GPU.  Exit status 0 = all checks passed.
+#define FDMGPU_THROW 1
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+#include "fdm/lapl_cube.h"
+#include "fdm/lapl_rect.h"
+
+static int failures = 0;
+#define CHECK(cond, ...)                                  \
+    do {                                                  \
+        if (!(cond)) {                                    \
+            std::printf("FAIL %s:%d ", __FILE__, __LINE__); \
+            std::printf(__VA_ARGS__);                     \
+            std::printf("\n");                            \
+            failures++;                                   \
+        }                                                 \
+    } while (0)
+
+static double sq(double x) { return x * x; }
+
+// u = sin^2 x + cos^2 y + sin^2 z on the cell-centred grid of [0,1]^3 with the
+// boundary values moved to the right-hand side; the discretisation error of
+// the 7-point scheme at 31^3 is below 1e-4 (ut/ut_lapl_cube.cpp:39-125).
+template <typename T>
+double cube_analytic(int n, int mode) {
+    fdm::gpu::set_double_mode(mode);
+    const double h = 1.0 / n;
+    auto ans = [&](int i, int k, int j) {
+        double x = h * j - h / 2, y = h * k - h / 2, z = h * i - h / 2;
+        return sq(std::sin(x)) + sq(std::cos(y)) + sq(std::sin(z));
+    };
+    auto rp = [&](int i, int k, int j) {
+        double x = h * j - h / 2, y = h * k - h / 2, z = h * i - h / 2;
+        return 2 * sq(std::sin(y)) - 2 * sq(std::cos(y)) - 2 * sq(std::sin(x)) + 2 * sq(std::cos(x)) -
+               2 * sq(std::sin(z)) + 2 * sq(std::cos(z));
+    };
+    fdm::LaplCube<T, true> lapl(h, h, h, 1 + h, 1 + h, 1 + h, n, n, n);
+    std::vector<T> rhs((size_t)n * n * n), sol(rhs.size());
+    for (int i = 1; i <= n; i++)
+        for (int k = 1; k <= n; k++)
+            for (int j = 1; j <= n; j++) {
+                double r = rp(i, k, j);
+                if (i == 1) r -= ans(i - 1, k, j) / (h * h);
+                if (k == 1) r -= ans(i, k - 1, j) / (h * h);
+                if (j == 1) r -= ans(i, k, j - 1) / (h * h);
+                if (j == n) r -= ans(i, k, j + 1) / (h * h);
+                if (k == n) r -= ans(i, k + 1, j) / (h * h);
+                if (i == n) r -= ans(i + 1, k, j) / (h * h);
+                rhs[((size_t)(i - 1) * n + (k - 1)) * n + (j - 1)] = (T)r;
+            }
+    lapl.solve(sol.data(), rhs.data());
+    double err = 0, nrm = 0;
+    for (int i = 1; i <= n; i++)
+        for (int k = 1; k <= n; k++)
+            for (int j = 1; j <= n; j++) {
+                double f = ans(i, k, j);
+                err = std::max(err, std::fabs(sol[((size_t)(i - 1) * n + (k - 1)) * n + (j - 1)] - f));
+                nrm = std::max(nrm, std::fabs(f));
+            }
+    return err / nrm;
+}
+
+// biquadratic u = (x-x1)(x-x2)(y-y1)(y-y2): the 5-point scheme is exact
+// (ut/ut_lapl_rect.cpp:300-374): < 1e-14 double, < 1e-5 float.
+template <typename T>
+double rect_biquadratic(int n) {
+    const double h = 1.0 / n;
+    auto a0 = [&](int k, int j) {
+        double x = h * j - h / 2, y = h * k - h / 2;
+        return x * (x - 1) * y * (y - 1);
+    };
+    fdm::LaplRect<T, true> lapl(h, h, 1 + h, 1 + h, n, n);
+    std::vector<T> rhs((size_t)n * n), sol(rhs.size());
+    for (int k = 1; k <= n; k++)
+        for (int j = 1; j <= n; j++) {
+            double x = h * j - h / 2, y = h * k - h / 2;
+            double r = 2 * y * (y - 1) + 2 * x * (x - 1);
+            if (k == 1) r -= a0(k - 1, j) / (h * h);
+            if (j == 1) r -= a0(k, j - 1) / (h * h);
+            if (j == n) r -= a0(k, j + 1) / (h * h);
+            if (k == n) r -= a0(k + 1, j) / (h * h);
+            rhs[(size_t)(k - 1) * n + (j - 1)] = (T)r;
+        }
+    lapl.solve(sol.data(), rhs.data());
+    double err = 0, nrm = 0;
+    for (int k = 1; k <= n; k++)
+        for (int j = 1; j <= n; j++) {
+            err = std::max(err, std::fabs(sol[(size_t)(k - 1) * n + (j - 1)] - a0(k, j)));
+            nrm = std::max(nrm, std::fabs(a0(k, j)));
+        }
+    return err / nrm;
+}
+
+int main() {
+    double e;
+    e = cube_analytic<double>(31, FDMGPU_F64);
+    std::printf("LaplCube<double> 31^3 analytic (f64): err %.3e\n", e);
+    CHECK(e < 1e-4, "cube double err %g", e);
+    e = cube_analytic<double>(31, FDMGPU_F32_REFINE64);
+    std::printf("LaplCube<double> 31^3 analytic (refine): err %.3e\n", e);
+    CHECK(e < 1e-4, "cube double refine err %g", e);
+    e = cube_analytic<float>(31, FDMGPU_F64);
+    std::printf("LaplCube<float> 31^3 analytic: err %.3e\n", e);
+    CHECK(e < 1e-4, "cube float err %g", e);
+    e = rect_biquadratic<double>(31);
+    std::printf("LaplRect<double> 31^2 biquadratic: err %.3e\n", e);
+    CHECK(e < 1e-14, "rect double err %g", e);
+    e = rect_biquadratic<float>(31);
+    std::printf("LaplRect<float> 31^2 biquadratic: err %.3e\n", e);
+    CHECK(e < 1e-5, "rect float err %g", e);
+
+    // the reference aborts for a non-power-of-two transform length (src/fft.cpp:67);
+    // with FDMGPU_THROW the shim throws instead
+    bool threw = false;
+    try {
+        fdm::LaplCube<double, false> bad(0.1, 0.1, 0.1, 1, 1, 1, 6, 7, 7);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw, "non-power-of-two size must fail");
+
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);
+    return failures ? 1 : 0;
+}
