@@ -69,16 +69,19 @@ def algorithmic_bytes(kernel: str, mode: str, iters: int, n: int, batch: int) ->
     """Algorithmic HBM bytes per solve of `batch` problems for one kernel class.
 
     Per point: FP64 plain: xy_fwd 8+8, zcr 8+8, xy_inv 8+8.
-    FP32+refine, iteration 1: xy_fwd f64->f32 8+4, zcr 4+4, xy_inv f32->f64 4+8,
-    residual (read f, u; write r32) 8+8+4; iterations >= 2: xy_fwd 4+4, zcr 4+4,
-    xy_inv with update (read c, u; write u) 4+8+8, residual 8+8+4.
+    FP32+refine with k inner solves (cubes whose planes fit LDS):
+      solve 1:      xy_fwd f64->f32 8+4, zcr 4+4, xy_inv f32->f64 4+8;
+      solve i >= 2: xy_fwd with the fused FP64 residual (read f, u; write r32) 8+8+4,
+                    zcr 4+4, xy_inv with update (read c, u; write u) 4+8+8,
+                    norm-only residual certification (read f, u) 8+8.
+    Total 32 + 48 (k-1) bytes/point (96 for k = 2, the 48 k n model).
     """
     pts = float(n) ** 3 * batch
     if mode == "f64":
         per = {"xy_fwd": 16, "zcr": 16, "xy_inv": 16}.get(kernel, 0)
         return per * pts
-    first = {"xy_fwd": 12, "zcr": 8, "xy_inv": 12, "residual": 20}.get(kernel, 0)
-    later = {"xy_fwd": 8, "zcr": 8, "xy_inv": 20, "residual": 20}.get(kernel, 0)
+    first = {"xy_fwd": 12, "zcr": 8, "xy_inv": 12, "residual": 0}.get(kernel, 0)
+    later = {"xy_fwd": 20, "zcr": 8, "xy_inv": 20, "residual": 16}.get(kernel, 0)
     return (first + later * max(0, iters - 1)) * pts
 
 

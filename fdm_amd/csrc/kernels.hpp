@@ -58,8 +58,26 @@ struct RowCrArgs {
     const int* done;
 };
 
+// Forward plane DST of r = f - A u (FP64 residual fused into the load) to FP32,
+// with per-plane (sum r^2, sum f^2) partials for the convergence check.
+struct PlaneResidArgs {
+    const double* f;
+    const void* u;        // FP64 iterate
+    float* out;
+    double* partial;      // [batch][nplanes][2]
+    long bs;
+    int nplanes, batch;
+    double ix, iy, iz;    // 1/h^2 of the inverted operator per axis
+    double sx, sy;
+    const void* tables;   // FP32 DST tables
+    hipStream_t stream;
+    const int* done;
+};
+
 template <typename Tin, typename Tout, typename T>
 hipError_t launch_plane(int logn, const PlaneArgs& a);
+template <typename Tu>
+hipError_t launch_plane_resid(int logn, const PlaneResidArgs& a);
 template <typename T>
 bool plane_fits(int logn);
 template <typename Tin, typename Tout, typename T>

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
     const long elems = (long)nx * ny * nz;
     const double* fp = f + p * bs;
     const double* up = u + p * bs;
-    float* rp = r32 + p * bs;
+    float* rp = r32 ? r32 + p * bs : nullptr;
     const long pl = (long)nx * ny;
     double rr = 0.0, ff = 0.0;
     for (long idx = (long)blockIdx.x * kResBlock + threadIdx.x; idx < elems; idx += (long)gridDim.x * kResBlock) {
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
         }
         const double fv = fp[idx];
         const double r = fv - au;
-        rp[idx] = (float)r;
+        if (r32) rp[idx] = (float)r;
         rr += r * r;
         ff += fv * fv;
     }
@@ -130,6 +130,12 @@ hipError_t launch_residual(const ResidualArgs& a) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(check_kernel, dim3(a.batch), dim3(kResBlock), 0, a.stream, a.partial, (int)nblk, a.tol, a.done,
                        a.iters, a.resid, a.it);
+    return hipGetLastError();
+}
+
+hipError_t launch_check(const double* partial, int nblk, double tol, int* done, int* iters, double* resid, int it,
+                        int batch, hipStream_t s) {
+    hipLaunchKernelGGL(check_kernel, dim3(batch), dim3(kResBlock), 0, s, partial, nblk, tol, done, iters, resid, it);
     return hipGetLastError();
 }
 

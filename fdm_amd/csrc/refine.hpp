@@ -27,6 +27,9 @@ struct ResidualArgs {
 };
 
 hipError_t launch_residual(const ResidualArgs& a);
+// norm check over `nblk` partials per problem (fixed order); see refine.hip
+hipError_t launch_check(const double* partial, int nblk, double tol, int* done, int* iters, double* resid, int it,
+                        int batch, hipStream_t s);
 long residual_partials(int nx, int ny, int nz);
 hipError_t launch_fill_int(int* a, int n, int v, hipStream_t s);
 
