@@ -37,7 +37,8 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_rect_set_scales", "fd
 
 # kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
 K_XY_FWD, K_XY_INV, K_X_TILE, K_Y_TILE, K_ZCR, K_ROWCR, K_RESIDUAL, K_OTHER = range(8)
-KERNEL_NAMES = ("xy_fwd", "xy_inv", "x_tile", "y_tile", "zcr", "rowcr", "residual", "other")
+KERNEL_NAMES = ("xy_fwd", "xy_inv", "x_tile", "y_tile", "zcr", "rowcr", "residual", "other", "xy_fwd_resid",
+                "xy_inv_acc")
 
 
 class FdmGpuError(RuntimeError):

@@ -247,7 +247,7 @@ int stage_inv(fdmgpu_plan_t* P, void* in, void* out, int count, hipStream_t s, b
     const double sx = std::sqrt(2. / P->lx), sy = std::sqrt(2. / P->ly);
     if (P->kind == 3 && P->logx == P->logy && plane_fits<T>(P->logx)) {
         PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done};
-        Timed tm(P, FDMGPU_K_XY_INV, s);
+        Timed tm(P, accum ? FDMGPU_K_XY_INV_ACC : FDMGPU_K_XY_INV, s);
         HIPCHK((launch_plane<T, Tout, T>(P->logx, a)));
         return FDMGPU_OK;
     }
@@ -372,7 +372,7 @@ int solve_refine(fdmgpu_plan_t* P, double* ans, const double* rhs, int count, hi
             PlaneResidArgs pa{rhs, ans, P->r32, P->partial, elems, P->nz, count, ra.ix, ra.iy, ra.iz,
                               P->dx * std::sqrt(2. / P->lx), P->dy * std::sqrt(2. / P->ly), P->tab_x, s, P->done};
             {
-                Timed tm(P, FDMGPU_K_XY_FWD, s);
+                Timed tm(P, FDMGPU_K_XY_FWD_RESID, s);
                 HIPCHK(launch_plane_resid<double>(P->logx, pa));
             }
             if (it == 2) {   // the iterate after solve 1 was not certified

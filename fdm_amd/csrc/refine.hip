@@ -35,9 +35,13 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* sh) {
     b = sh[kResBlock];
 }
 
-// r = f - A u (FP64), r32 = (float) r; partial[p][blk] = (sum r^2, sum f^2).
-// Grid-stride over the points of problem blockIdx.y with a capped grid, so a
-// launch for already-converged problems costs a few microseconds.
+// r = f - A u (FP64), r32 = (float) r; partial[p][c] = (sum r^2, sum f^2)
+// over chunk c of problem blockIdx.y.  A chunk is a contiguous range of rows
+// (row = z * ny + y), one wave per row and the lanes along x.  The gridDim.x
+// chunks (a multiple of 8) are dealt to the 8 XCDs round-robin, so
+// chunk = (bid % 8) * (nblk / 8) + bid / 8 gives every XCD one contiguous slab
+// of z-planes and the u planes at z +- 1 are re-read from that XCD's L2.
+// A launch for already-converged problems costs a few microseconds.
 template <bool THREE, bool SCALED>
 __global__ __launch_bounds__(kResBlock) void residual_kernel(
     const double* __restrict__ f, const double* __restrict__ u, float* __restrict__ r32,
@@ -47,40 +51,47 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
     __shared__ double sh[2 * kResBlock];
     const int p = blockIdx.y;
     if (done && done[p]) return;
-    const long elems = (long)nx * ny * nz;
     const double* fp = f + p * bs;
     const double* up = u + p * bs;
     float* rp = r32 ? r32 + p * bs : nullptr;
     const long pl = (long)nx * ny;
+    const int nblk = gridDim.x;
+    const int c = (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;
+    const int nrows = ny * nz;
+    const int chunk = (nrows + nblk - 1) / nblk;
+    const int row_end = min(nrows, (c + 1) * chunk);
+    const int lane = threadIdx.x & 63;
     double rr = 0.0, ff = 0.0;
-    for (long idx = (long)blockIdx.x * kResBlock + threadIdx.x; idx < elems; idx += (long)gridDim.x * kResBlock) {
-        const int x = (int)(idx % nx);
-        const long t = idx / nx;
-        const int y = (int)(t % ny);
-        const int z = (int)(t / ny);
-        const double uc = up[idx];
-        const double xm = x > 0 ? up[idx - 1] : 0.0, xp = x < nx - 1 ? up[idx + 1] : 0.0;
-        const double ym = y > 0 ? up[idx - nx] : 0.0, yp = y < ny - 1 ? up[idx + nx] : 0.0;
-        double au;
-        if constexpr (SCALED) {
-            au = (lx[x] * xm - 2.0 * uc + ux[x] * xp) * ix + (ym - 2.0 * uc + yp) * iy * sy[x];
-        } else {
-            au = (xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy;
+    for (int row = c * chunk + (int)(threadIdx.x >> 6); row < row_end; row += kResBlock / 64) {
+        const int y = row % ny;
+        const int z = row / ny;
+        const long base = (long)row * nx;
+        for (int x = lane; x < nx; x += 64) {
+            const long idx = base + x;
+            const double uc = up[idx];
+            const double xm = x > 0 ? up[idx - 1] : 0.0, xp = x < nx - 1 ? up[idx + 1] : 0.0;
+            const double ym = y > 0 ? up[idx - nx] : 0.0, yp = y < ny - 1 ? up[idx + nx] : 0.0;
+            double au;
+            if constexpr (SCALED) {
+                au = (lx[x] * xm - 2.0 * uc + ux[x] * xp) * ix + (ym - 2.0 * uc + yp) * iy * sy[x];
+            } else {
+                au = (xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy;
+            }
+            if constexpr (THREE) {
+                const double zm = z > 0 ? up[idx - pl] : 0.0, zp = z < nz - 1 ? up[idx + pl] : 0.0;
+                au += (zm - 2.0 * uc + zp) * iz;
+            }
+            const double fv = fp[idx];
+            const double r = fv - au;
+            if (r32) rp[idx] = (float)r;
+            rr += r * r;
+            ff += fv * fv;
         }
-        if constexpr (THREE) {
-            const double zm = z > 0 ? up[idx - pl] : 0.0, zp = z < nz - 1 ? up[idx + pl] : 0.0;
-            au += (zm - 2.0 * uc + zp) * iz;
-        }
-        const double fv = fp[idx];
-        const double r = fv - au;
-        if (r32) rp[idx] = (float)r;
-        rr += r * r;
-        ff += fv * fv;
     }
     block_sum2<double>(rr, ff, sh);
     if (threadIdx.x == 0) {
-        partial[((long)p * gridDim.x + blockIdx.x) * 2 + 0] = rr;
-        partial[((long)p * gridDim.x + blockIdx.x) * 2 + 1] = ff;
+        partial[((long)p * nblk + c) * 2 + 0] = rr;
+        partial[((long)p * nblk + c) * 2 + 1] = ff;
     }
 }
 
@@ -139,9 +150,13 @@ hipError_t launch_check(const double* partial, int nblk, double tol, int* done, 
     return hipGetLastError();
 }
 
+// chunks per problem: about 4 rows each, a multiple of 8 (one slab per XCD),
+// at most kResMaxBlocks
 long residual_partials(int nx, int ny, int nz) {
-    const long elems = (long)nx * ny * nz;
-    const long b = (elems + kResBlock - 1) / kResBlock;
+    (void)nx;
+    const long rows = (long)ny * nz;
+    long b = (rows + 3) / 4;
+    b = (b + 7) / 8 * 8;
     return b < kResMaxBlocks ? b : kResMaxBlocks;
 }
 

@@ -119,7 +119,9 @@ int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resid
 #define FDMGPU_K_ROWCR 5    /* LaplRect row cyclic reduction */
 #define FDMGPU_K_RESIDUAL 6 /* FP64 residual + norm + convergence flag */
 #define FDMGPU_K_OTHER 7    /* flag fills, FP32->FP64 updates, copies */
-#define FDMGPU_NUM_KERNELS 8
+#define FDMGPU_K_XY_FWD_RESID 8 /* FP64 residual fused into the forward plane DST (refinement solves >= 2) */
+#define FDMGPU_K_XY_INV_ACC 9   /* inverse plane DST adding the correction into the FP64 iterate */
+#define FDMGPU_NUM_KERNELS 10
 int fdmgpu_set_timing(fdmgpu_plan plan, int enable);
 int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launches);
 
