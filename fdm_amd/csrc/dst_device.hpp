@@ -26,6 +26,8 @@
 
 #include <type_traits>
 
+#include "kernels.hpp"   // kDst, kRdftFwd, kRdftInv
+
 namespace fdmgpu {
 
 template <int B, int E, typename F>
@@ -216,14 +218,45 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// DST-I of 2*JOBS pencils in LDS: pencil p, slot j at lds[Lay::addr(p, j)];
-// job g transforms pencils g and g + JOBS.  On entry slots 1..N-1 hold
-// x_1..x_{N-1} in natural order and slot 0 holds a finite value; on exit slot
-// sigma(j) holds scale*X_j (j = 1..N-1) and slot 0 holds 0.  All NT threads of
-// the block call this; no block barrier inside (jobs never straddle a wavefront).
-template <typename T, int LOGN, int JOBS, typename Lay, int NT>
-__device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, LOGN>& L, const T scale,
-                                          const int tid) {
+// Pencil transform kinds.
+//   kDst:     DST-I, FFT<T>::sFFT (reference src/fft.cpp:293-365); N-1 points
+//             in slots 1..N-1 (slot 0 is a dummy).
+//   kRdftFwd: half-complex real DFT, FFT<T>::pFFT_1 (src/fft.cpp:108-193):
+//             X[0] = s*sum x_j, X[k] = s*sum x_j cos(2 pi k j/N),
+//             X[N-k] = s*sum x_j sin(2 pi k j/N) (0 < k < N/2),
+//             X[N/2] = s*sum (-1)^j x_j; N points in slots 0..N-1.
+//   kRdftInv: its inverse series, FFT<T>::pFFT (src/fft.cpp:195-212):
+//             x_j = s*(X[0]/2 + X[N/2](-1)^j/2
+//                   + sum_{0<k<N/2} X[k] cos(2 pi k j/N) + X[N-k] sin(2 pi k j/N)).
+// pFFT(pFFT_1(x)) = (N/2) x, the same round-trip factor as the DST-I.
+
+template <int KIND, int LOGN>
+struct PencilIO {
+    using S = DstShape<LOGN>;
+    static constexpr int N = S::N;
+    static constexpr int n = KIND == kDst ? N - 1 : N;   // points per pencil
+    static constexpr int off = KIND == kDst ? 1 : 0;     // slot of point 0 on input
+    // slot that holds output point m (0 <= m < n) after the transform
+    __device__ static constexpr int out(int m) { return KIND == kDst ? S::sigma(m + 1) : S::posZ(m); }
+};
+
+// Transform of 2*JOBS pencils in LDS: pencil p, slot j at lds[Lay::addr(p, j)];
+// job g transforms pencils g and g + JOBS (packed as the real and imaginary
+// parts of one complex FFT of length N).  On entry the points sit in natural
+// order from slot PencilIO::off (kDst: slot 0 holds a finite value); on exit
+// output point m sits in slot PencilIO::out(m), times `scale` (kDst: slot 0
+// holds 0).  All NT threads of the block call this; no block barrier inside
+// (jobs never straddle a wavefront).
+//
+// kRdftFwd: Z = FFT(a + i b); A_k = (Z_k + conj Z_{N-k})/2,
+//   B_k = (Z_k - conj Z_{N-k})/(2i), X_a[k] = Re A_k, X_a[N-k] = -Im A_k.
+//   Step 3 reads Z_k, Z_{N-k} and writes X[k], X[N-k] into the same two slots.
+// kRdftInv: C_k = X[k]/2 - i X[N-k]/2 (Hermitian; C_0, C_{N/2} real),
+//   w = C^a + i C^b, x^a + i x^b = sum_k w_k e^{+2 pi i jk/N}
+//   = conj(FFT(conj w)); step 2 writes the result directly.
+template <int KIND, typename T, int LOGN, int JOBS, typename Lay, int NT>
+__device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<T, LOGN>& L, const T scale,
+                                             const int tid) {
     using S = DstShape<LOGN>;
     constexpr int N = S::N, N1 = S::N1, N2 = S::N2, KC = S::KC;
     constexpr int WORK = JOBS * N1;
@@ -243,12 +276,27 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, 
 #pragma unroll
             for (int j2 = 0; j2 < N2; ++j2) {
                 const int j = t + N1 * j2;
-                const int jn = j2 == 0 ? ((N - t) & (N - 1)) : N - j;   // slot 0 for j = 0
-                const T a0 = PA(j), a1 = PA(jn);
-                const T b0 = PB(j), b1 = PB(jn);
-                const T s = L.sn[j2];
-                yr[j2] = s * (a0 + a1) + T(0.5) * (a0 - a1);
-                yi[j2] = s * (b0 + b1) + T(0.5) * (b0 - b1);
+                if constexpr (KIND == kRdftFwd) {
+                    yr[j2] = PA(j);
+                    yi[j2] = PB(j);
+                } else {
+                    const int jn = j2 == 0 ? ((N - t) & (N - 1)) : N - j;   // slot 0 for j = 0
+                    const T a0 = PA(j), a1 = PA(jn);
+                    const T b0 = PB(j), b1 = PB(jn);
+                    if constexpr (KIND == kDst) {
+                        const T s = L.sn[j2];
+                        yr[j2] = s * (a0 + a1) + T(0.5) * (a0 - a1);
+                        yi[j2] = s * (b0 + b1) + T(0.5) * (b0 - b1);
+                    } else if (2 * j2 < N2) {   // 0 <= j < N/2
+                        const bool edge = j == 0;
+                        yr[j2] = edge ? T(0.5) * a0 : T(0.5) * (a0 + b1);
+                        yi[j2] = edge ? T(-0.5) * b0 : T(0.5) * (a1 - b0);
+                    } else {                              // N/2 <= j < N
+                        const bool edge = j == N / 2;
+                        yr[j2] = edge ? T(0.5) * a0 : T(0.5) * (a1 - b0);
+                        yi[j2] = edge ? T(-0.5) * b0 : T(-0.5) * (a0 + b1);
+                    }
+                }
             }
             dft<N2>(yr, yi);
 #pragma unroll
@@ -294,60 +342,89 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, 
 #pragma unroll
                 for (int k1 = 0; k1 < N1; ++k1) {
                     const int pos = S::posZ(t + N1 * r + N2 * k1);
-                    PA(pos) = zr[r * N1 + k1];
-                    PB(pos) = zi[r * N1 + k1];
+                    if constexpr (KIND == kRdftInv) {   // x^a = Re Y, x^b = -Im Y
+                        PA(pos) = zr[r * N1 + k1] * scale;
+                        PB(pos) = -zi[r * N1 + k1] * scale;
+                    } else {
+                        PA(pos) = zr[r * N1 + k1];
+                        PB(pos) = zi[r * N1 + k1];
+                    }
                 }
             }
             wave_lds_sync();
 
-            // ---- step 3: even outputs directly, odd outputs by prefix sum
-            T ae[KC], ao[KC], be[KC], bo[KC];
-            T runA = T(0), runB = T(0);
-#pragma unroll
-            for (int i = 0; i < KC; ++i) {
-                const int k = t * KC + i;
-                const int pk = S::posZ(k), pn = S::posZ((N - k) & (N - 1));
-                const T zkr = PA(pk), zki = PB(pk);
-                const T znr = PA(pn), zni = PB(pn);
-                ae[i] = T(0.5) * (zni - zki);
-                be[i] = T(0.5) * (zkr - znr);
-                T dA = T(0.5) * (zkr + znr);
-                T dB = T(0.5) * (zki + zni);
-                if (k == 0) {
-                    dA *= T(0.5);
-                    dB *= T(0.5);
-                }
-                runA += dA;
-                runB += dB;
-                ao[i] = runA;
-                bo[i] = runB;
-            }
-            if constexpr (N1 > 1) {
-                const T offA = lane_scan_incl<N1>(runA) - runA;
-                const T offB = lane_scan_incl<N1>(runB) - runB;
+            if constexpr (KIND == kRdftFwd) {
+                // ---- step 3: split the packed spectra; lane t owns k in [t*KC, (t+1)*KC)
+                // and their mirrors N-k (k = 0: the Nyquist slot N/2)
 #pragma unroll
                 for (int i = 0; i < KC; ++i) {
-                    ao[i] += offA;
-                    bo[i] += offB;
+                    const int k = t * KC + i;
+                    const bool k0 = k == 0;
+                    const int pk = S::posZ(k), pn = S::posZ(k0 ? N / 2 : N - k);
+                    const T zkr = PA(pk), zki = PB(pk);
+                    const T znr = PA(pn), zni = PB(pn);
+                    PA(pk) = (k0 ? zkr : T(0.5) * (zkr + znr)) * scale;
+                    PB(pk) = (k0 ? zki : T(0.5) * (zki + zni)) * scale;
+                    PA(pn) = (k0 ? znr : T(0.5) * (zni - zki)) * scale;
+                    PB(pn) = (k0 ? zni : T(0.5) * (zkr - znr)) * scale;
                 }
-            }
-            wave_lds_sync();
+                wave_lds_sync();
+            } else if constexpr (KIND == kDst) {
+                // ---- step 3: even outputs directly, odd outputs by prefix sum
+                T ae[KC], ao[KC], be[KC], bo[KC];
+                T runA = T(0), runB = T(0);
 #pragma unroll
-            for (int i = 0; i < KC; ++i) {
-                const int k = t * KC + i;
-                const int pe = S::sigma(2 * k), po = S::sigma(2 * k + 1);
-                const T e_a = k == 0 ? T(0) : ae[i] * scale;
-                const T e_b = k == 0 ? T(0) : be[i] * scale;
-                PA(pe) = e_a;
-                PB(pe) = e_b;
-                PA(po) = ao[i] * scale;
-                PB(po) = bo[i] * scale;
+                for (int i = 0; i < KC; ++i) {
+                    const int k = t * KC + i;
+                    const int pk = S::posZ(k), pn = S::posZ((N - k) & (N - 1));
+                    const T zkr = PA(pk), zki = PB(pk);
+                    const T znr = PA(pn), zni = PB(pn);
+                    ae[i] = T(0.5) * (zni - zki);
+                    be[i] = T(0.5) * (zkr - znr);
+                    T dA = T(0.5) * (zkr + znr);
+                    T dB = T(0.5) * (zki + zni);
+                    if (k == 0) {
+                        dA *= T(0.5);
+                        dB *= T(0.5);
+                    }
+                    runA += dA;
+                    runB += dB;
+                    ao[i] = runA;
+                    bo[i] = runB;
+                }
+                if constexpr (N1 > 1) {
+                    const T offA = lane_scan_incl<N1>(runA) - runA;
+                    const T offB = lane_scan_incl<N1>(runB) - runB;
+#pragma unroll
+                    for (int i = 0; i < KC; ++i) {
+                        ao[i] += offA;
+                        bo[i] += offB;
+                    }
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int i = 0; i < KC; ++i) {
+                    const int k = t * KC + i;
+                    const int pe = S::sigma(2 * k), po = S::sigma(2 * k + 1);
+                    const T e_a = k == 0 ? T(0) : ae[i] * scale;
+                    const T e_b = k == 0 ? T(0) : be[i] * scale;
+                    PA(pe) = e_a;
+                    PB(pe) = e_b;
+                    PA(po) = ao[i] * scale;
+                    PB(po) = bo[i] * scale;
+                }
+                wave_lds_sync();
             }
-            wave_lds_sync();
 #undef PA
 #undef PB
         }
     }
+}
+
+template <typename T, int LOGN, int JOBS, typename Lay, int NT>
+__device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, LOGN>& L, const T scale,
+                                          const int tid) {
+    pencil_pairs<kDst, T, LOGN, JOBS, Lay, NT>(lds, L, scale, tid);
 }
 
 // ---------------------------------------------------------------------------

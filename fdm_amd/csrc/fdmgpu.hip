@@ -12,6 +12,9 @@
 // The solve runs in place in `ans` (no scratch) for plain F32/F64 plans.
 // LaplRect (src/lapl_rect.cpp:63-110): y DST -> per-row cyclic reduction of
 // the gtsv system -> inverse y DST.
+// Periodic axes (tensor_flags<periodic...>) use the half-complex real DFT
+// pair (pFFT_1 / pFFT) instead of the DST, and the periodic cube's z systems
+// are circulant (see zcrp_kernel).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +34,7 @@ struct fdmgpu_plan_t {
     int kind = 3;          // 3 = LaplCube, 2 = LaplRect
     int precision = FDMGPU_F64;
     int periodic = 0;
+    int kfwd = 0, kinv = 0;             // pencil transform kinds (kDst, or kRdftFwd/kRdftInv if periodic)
     int nx = 0, ny = 0, nz = 1;
     int logx = 0, logy = 0, logz = 0;   // log2 of the transform lengths
     double dx = 1, dy = 1, dz = 1, lx = 1, ly = 1, lz = 1;
@@ -132,6 +136,19 @@ std::vector<double> dirichlet_lambda(int n, double h, bool as_float) {
     return l;
 }
 
+// lambda_k = 4/h^2 sin^2(k pi / n), k = 0..n-1 (periodic axis, init_lm with
+// tensor_flag::periodic, src/lapl_cube.cpp:145-172): the eigenvalue of
+// half-complex coefficient k (cosine k <= n/2, sine of frequency n-k above).
+std::vector<double> periodic_lambda(int n, double h, bool as_float) {
+    std::vector<double> l(n);
+    for (int k = 0; k < n; k++) {
+        const double s = std::sin(k * M_PI / n);
+        const double v = 4. / (h * h) * (s * s);
+        l[k] = as_float ? (double)(float)v : v;
+    }
+    return l;
+}
+
 bool valid_precision(int p) { return p == FDMGPU_F32 || p == FDMGPU_F64 || p == FDMGPU_F32_REFINE64; }
 
 int alloc_tables(fdmgpu_plan_t* P) {
@@ -217,24 +234,25 @@ int stage_fwd(fdmgpu_plan_t* P, const void* in, void* out, int count, hipStream_
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = P->dx * std::sqrt(2. / P->lx), sy = P->dy * std::sqrt(2. / P->ly);
     if (P->kind == 3 && P->logx == P->logy && plane_fits<T>(P->logx)) {
-        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done};
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done, P->kfwd};
         Timed tm(P, FDMGPU_K_XY_FWD, s);
         HIPCHK((launch_plane<Tin, T, T>(P->logx, a)));
         return FDMGPU_OK;
     }
     if (P->kind == 3) {
-        AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, false, done};
+        AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, false, done,
+                    P->kfwd};
         {
             Timed tm(P, FDMGPU_K_X_TILE, s);
             HIPCHK((launch_xaxis<Tin, T, T>(P->logx, ax)));
         }
-        AxisArgs ay{out, out, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done};
+        AxisArgs ay{out, out, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd};
         Timed tm(P, FDMGPU_K_Y_TILE, s);
         HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
         return FDMGPU_OK;
     }
     // rect: y only
-    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done};
+    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd};
     Timed tm(P, FDMGPU_K_Y_TILE, s);
     HIPCHK((launch_yaxis<Tin, T, T>(P->logy, ay)));
     return FDMGPU_OK;
@@ -246,34 +264,40 @@ int stage_inv(fdmgpu_plan_t* P, void* in, void* out, int count, hipStream_t s, b
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = std::sqrt(2. / P->lx), sy = std::sqrt(2. / P->ly);
     if (P->kind == 3 && P->logx == P->logy && plane_fits<T>(P->logx)) {
-        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done};
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv};
         Timed tm(P, accum ? FDMGPU_K_XY_INV_ACC : FDMGPU_K_XY_INV, s);
         HIPCHK((launch_plane<T, Tout, T>(P->logx, a)));
         return FDMGPU_OK;
     }
     if (P->kind == 3) {
-        AxisArgs ay{in, in, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done};
+        AxisArgs ay{in, in, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv};
         {
             Timed tm(P, FDMGPU_K_Y_TILE, s);
             HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
         }
-        AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, accum, done};
+        AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, accum, done,
+                    P->kinv};
         Timed tm(P, FDMGPU_K_X_TILE, s);
         HIPCHK((launch_xaxis<T, Tout, T>(P->logx, ax)));
         return FDMGPU_OK;
     }
     if (accum) {   // rect: inverse y into the FP32 buffer, then accumulate through an x pass-free add
-        AxisArgs ay{in, in, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done};
+        AxisArgs ay{in, in, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv};
         Timed tm(P, FDMGPU_K_Y_TILE, s);
         HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
         return -1;   // caller adds
     }
-    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done};
+    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv};
     Timed tm(P, FDMGPU_K_Y_TILE, s);
     HIPCHK((launch_yaxis<T, Tout, T>(P->logy, ay)));
     return FDMGPU_OK;
 }
 
+// FP32 plans (LaplCube<float>, LaplRect<float>) eliminate in double: the
+// systems of the lowest modes are nearly singular and an FP32 elimination
+// would lose ~cond * eps there, where the reference's z transforms keep
+// float accuracy.  The inner solves of FP32+refine stay FP32 (refinement
+// recovers the accuracy).
 template <typename T>
 int stage_tridiag(fdmgpu_plan_t* P, void* data, int count, hipStream_t s, const int* done) {
     const long elems = (long)P->nx * P->ny * P->nz;
@@ -281,12 +305,14 @@ int stage_tridiag(fdmgpu_plan_t* P, void* data, int count, hipStream_t s, const 
         const double b = 1.0 / (P->hz_op * P->hz_op);
         const double gamma = P->dz * (double)(1 << P->logz) / P->lz;
         ZcrArgs z{data, (long)P->nx * P->ny, P->nx, P->lmx, P->lmy, b, -2.0 * b,
-                  std::fabs(gamma - 1.0) < 4e-16 ? 1.0 : gamma, elems, count, s, done};
+                  std::fabs(gamma - 1.0) < 4e-16 ? 1.0 : gamma, elems, count, s, done, P->periodic != 0,
+                  P->precision == FDMGPU_F32};
         Timed tm(P, FDMGPU_K_ZCR, s);
         HIPCHK(launch_zcr<T>(P->logz, z));
         return FDMGPU_OK;
     }
-    RowCrArgs r{data, P->nx, P->ny, P->lmy, P->d_sc, P->d_Ls, P->d_Us, 1.0 / (P->dx * P->dx), elems, count, s, done};
+    RowCrArgs r{data, P->nx, P->ny, P->lmy, P->d_sc, P->d_Ls, P->d_Us, 1.0 / (P->dx * P->dx), elems, count, s, done,
+                P->precision == FDMGPU_F32};
     Timed tm(P, FDMGPU_K_ROWCR, s);
     HIPCHK(launch_rowcr<T>(r));
     return FDMGPU_OK;
@@ -474,7 +500,7 @@ int set_scales_impl(fdmgpu_plan_t* P, const double* s, const double* L, const do
     }
     // the GPU eliminates without row interchanges: require diagonal dominance of
     // every row system (the default scales and the cylindrical ones satisfy it)
-    std::vector<double> lam = dirichlet_lambda(P->ny, P->dy, false);
+    std::vector<double> lam = P->periodic ? periodic_lambda(P->ny, P->dy, false) : dirichlet_lambda(P->ny, P->dy, false);
     const double i2 = 1.0 / (P->dx * P->dx);
     for (int k = 0; k < P->ny; k++) {
         for (int j = 0; j < nx; j++) {
@@ -530,15 +556,22 @@ int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double
     *plan = nullptr;
     if (periodic_mask != 0 && periodic_mask != 7) return FDMGPU_ERR_INVALID_ARG;
     if (nx < 1 || ny < 1 || nz < 1) return FDMGPU_ERR_INVALID_SIZE;
-    if (periodic_mask) return FDMGPU_ERR_UNSUPPORTED;
-    const int lgx = ilog2_exact(nx + 1), lgy = ilog2_exact(ny + 1), lgz = ilog2_exact(nz + 1);
+    const bool per = periodic_mask != 0;
+    // transform lengths: n+1 on Dirichlet axes, n on periodic ones (src/lapl_cube.h:66-76)
+    const int e = per ? 0 : 1;
+    const int lgx = ilog2_exact(nx + e), lgy = ilog2_exact(ny + e), lgz = ilog2_exact(nz + e);
     if (lgx < 0 || lgy < 0 || lgz < 0) return FDMGPU_ERR_INVALID_SIZE;
     if (lgx > 9 || lgy > 9 || lgz > 9 || lgx < 1 || lgy < 1 || lgz < 1) return FDMGPU_ERR_UNSUPPORTED;
+    // the residual of a periodic (singular) operator is not implemented for refinement
+    if (per && precision == FDMGPU_F32_REFINE64) return FDMGPU_ERR_UNSUPPORTED;
     fdmgpu_plan_t* P = new (std::nothrow) fdmgpu_plan_t();
     if (!P) return FDMGPU_ERR_OUT_OF_MEMORY;
     int rc = common_init(P, precision);
     if (rc) { free_plan(P); return rc; }
     P->kind = 3;
+    P->periodic = periodic_mask;
+    P->kfwd = per ? kRdftFwd : kDst;
+    P->kinv = per ? kRdftInv : kDst;
     P->nx = nx; P->ny = ny; P->nz = nz;
     P->logx = lgx; P->logy = lgy; P->logz = lgz;
     P->dx = dx; P->dy = dy; P->dz = dz; P->lx = lx; P->ly = ly; P->lz = lz;
@@ -553,8 +586,9 @@ int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double
     P->hz_op = (nz == ny) ? dy : dz;
     const bool f32 = precision != FDMGPU_F64;
     const bool tf = precision == FDMGPU_F32;   // the reference's lambda tables are stored as T
-    std::vector<double> lmy = dirichlet_lambda(ny, dy, tf);
-    std::vector<double> lmx = (nx == ny) ? lmy : dirichlet_lambda(nx, dx, tf);
+    auto lambda = [&](int n, double h) { return per ? periodic_lambda(n, h, tf) : dirichlet_lambda(n, h, tf); };
+    std::vector<double> lmy = lambda(ny, dy);
+    std::vector<double> lmx = (nx == ny) ? lmy : lambda(nx, dx);
     if ((rc = alloc_tables(P)) ||
         (rc = f32 ? upload<float>(&P->lmx, lmx) : upload<double>(&P->lmx, lmx)) ||
         (rc = f32 ? upload<float>(&P->lmy, lmy) : upload<double>(&P->lmy, lmy))) {
@@ -571,15 +605,19 @@ int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double
     *plan = nullptr;
     if (periodic_mask != 0 && periodic_mask != FDMGPU_PERIODIC_Y) return FDMGPU_ERR_INVALID_ARG;
     if (nx < 1 || ny < 1) return FDMGPU_ERR_INVALID_SIZE;
-    if (periodic_mask) return FDMGPU_ERR_UNSUPPORTED;
-    const int lgy = ilog2_exact(ny + 1);
+    const bool per = periodic_mask != 0;
+    const int lgy = ilog2_exact(per ? ny : ny + 1);
     if (lgy < 0) return FDMGPU_ERR_INVALID_SIZE;
     if (lgy < 1 || lgy > 9 || nx > 4096) return FDMGPU_ERR_UNSUPPORTED;   // row CR stages 4*nx values in LDS
+    if (per && precision == FDMGPU_F32_REFINE64) return FDMGPU_ERR_UNSUPPORTED;
     fdmgpu_plan_t* P = new (std::nothrow) fdmgpu_plan_t();
     if (!P) return FDMGPU_ERR_OUT_OF_MEMORY;
     int rc = common_init(P, precision);
     if (rc) { free_plan(P); return rc; }
     P->kind = 2;
+    P->periodic = periodic_mask;
+    P->kfwd = per ? kRdftFwd : kDst;
+    P->kinv = per ? kRdftInv : kDst;
     P->nx = nx; P->ny = ny; P->nz = 1;
     P->logx = lgy; P->logy = lgy; P->logz = 0;
     P->dx = dx; P->dy = dy; P->lx = lx; P->ly = ly;
@@ -589,7 +627,8 @@ int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double
         return FDMGPU_ERR_UNSUPPORTED;
     }
     const bool f32 = precision != FDMGPU_F64;
-    std::vector<double> lmy = dirichlet_lambda(ny, dy, precision == FDMGPU_F32);
+    std::vector<double> lmy = per ? periodic_lambda(ny, dy, precision == FDMGPU_F32)
+                                  : dirichlet_lambda(ny, dy, precision == FDMGPU_F32);
     if ((rc = alloc_tables(P)) || (rc = f32 ? upload<float>(&P->lmy, lmy) : upload<double>(&P->lmy, lmy)) ||
         (rc = set_scales_impl(P, nullptr, nullptr, nullptr))) {
         free_plan(P);

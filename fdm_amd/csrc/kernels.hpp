@@ -5,6 +5,10 @@
 
 namespace fdmgpu {
 
+// Pencil transform kinds (see pencil_pairs in dst_device.hpp): DST-I on
+// Dirichlet axes, half-complex real DFT forward / inverse on periodic axes.
+enum : int { kDst = 0, kRdftFwd = 1, kRdftInv = 2 };
+
 struct PlaneArgs {
     const void* in;
     void* out;
@@ -15,6 +19,7 @@ struct PlaneArgs {
     hipStream_t stream;
     bool accum;           // out += result (refinement update) instead of out = result
     const int* done;      // per batch item: skip when non-zero (may be null)
+    int kind;             // kDst (0), kRdftFwd (1), kRdftInv (2); periodic kinds need Tin == Tout == T
 };
 
 struct AxisArgs {
@@ -29,6 +34,7 @@ struct AxisArgs {
     hipStream_t stream;
     bool accum;           // x kernel only
     const int* done;
+    int kind;             // as PlaneArgs::kind
 };
 
 struct ZcrArgs {
@@ -42,6 +48,8 @@ struct ZcrArgs {
     int batch;
     hipStream_t stream;
     const int* done;
+    bool periodic;        // circulant system of 2^q rows instead of 2^q - 1 rows
+    bool wide;            // float data, double arithmetic
 };
 
 struct RowCrArgs {
@@ -56,6 +64,7 @@ struct RowCrArgs {
     int batch;
     hipStream_t stream;
     const int* done;
+    bool wide;            // float data, double arithmetic
 };
 
 // Forward plane DST of r = f - A u (FP64 residual fused into the load) to FP32,

@@ -25,9 +25,12 @@ inline int& double_mode_ref() {
 }
 inline void set_double_mode(int precision) { double_mode_ref() = precision; }
 
+// Periodic solvers have a singular operator (zero mode), which the FP64
+// refinement residual does not handle: they always solve in plain FP64.
 template <typename T>
-inline int precision_for() {
-    return sizeof(T) == sizeof(float) ? FDMGPU_F32 : double_mode_ref();
+inline int precision_for(bool periodic = false) {
+    if (sizeof(T) == sizeof(float)) return FDMGPU_F32;
+    return periodic ? FDMGPU_F64 : double_mode_ref();
 }
 
 // The reference reports failures through verify(): message on stderr, then

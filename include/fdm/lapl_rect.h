@@ -34,7 +34,7 @@ public:
           sly(std::sqrt(2. / ly)), nx(nx), ny(ny), y1(periodic_y() ? 0 : 1), yn(periodic_y() ? ny - 1 : ny),
           ypoints(periodic_y() ? ny : ny + 1), lm_y_scale(nx + 1, T(1)), L_scale(nx + 1, T(1)),
           U_scale(nx + 1, T(1)), pushed_s_(nx + 1, T(1)), pushed_l_(nx + 1, T(1)), pushed_u_(nx + 1, T(1)) {
-        FDMGPU_VERIFY(fdmgpu_plan_rect(&plan_, gpu::precision_for<T>(), periodic_y() ? FDMGPU_PERIODIC_Y : 0, dx,
+        FDMGPU_VERIFY(fdmgpu_plan_rect(&plan_, gpu::precision_for<T>(periodic_y()), periodic_y() ? FDMGPU_PERIODIC_Y : 0, dx,
                                        dy, lx, ly, nx, ny));
     }
 

@@ -99,6 +99,40 @@ double rect_biquadratic(int n) {
     return err / nrm;
 }
 
+// periodic^3 on [0, 2pi)^3, n = 32: u = sin^3 x + cos^5 y - sin z with its
+// analytic Laplacian as rhs; the solve's mean is removed and the max error
+// must stay below 1e-2 (ut/ut_lapl_cube.cpp:134-244).
+template <typename T>
+double cube_periodic(int n) {
+    using flags = fdm::tensor_flags<fdm::tensor_flag::periodic, fdm::tensor_flag::periodic, fdm::tensor_flag::periodic>;
+    const double h = 2 * M_PI / n;
+    fdm::LaplCube<T, true, flags> lapl(h, h, h, 2 * M_PI, 2 * M_PI, 2 * M_PI, n, n, n);
+    auto coord = [&](int i) { return h * i - h / 2; };
+    std::vector<T> rhs((size_t)n * n * n), sol(rhs.size());
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++)
+            for (int j = 0; j < n; j++) {
+                const double x = coord(j), y = coord(k), z = coord(i);
+                rhs[((size_t)i * n + k) * n + j] =
+                    (T)(1. / 16. * (-12. * std::sin(x) + 36. * std::sin(3 * x) - 10. * std::cos(y) -
+                                    45. * std::cos(3 * y) - 25. * std::cos(5 * y) + 16. * std::sin(z)));
+            }
+    lapl.solve(sol.data(), rhs.data());
+    double avg = 0;
+    for (T v : sol) avg += v;
+    avg /= (double)sol.size();
+    double err = 0, nrm = 0;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++)
+            for (int j = 0; j < n; j++) {
+                const double x = coord(j), y = coord(k), z = coord(i);
+                const double f = std::pow(std::sin(x), 3) + std::pow(std::cos(y), 5) - std::sin(z);
+                err = std::max(err, std::fabs(sol[((size_t)i * n + k) * n + j] - avg - f));
+                nrm = std::max(nrm, std::fabs(f));
+            }
+    return err / nrm;
+}
+
 int main() {
     double e;
     e = cube_analytic<double>(31, FDMGPU_F64);
@@ -110,6 +144,12 @@ int main() {
     e = cube_analytic<float>(31, FDMGPU_F64);
     std::printf("LaplCube<float> 31^3 analytic: err %.3e\n", e);
     CHECK(e < 1e-4, "cube float err %g", e);
+    e = cube_periodic<double>(32);
+    std::printf("LaplCube<double, periodic^3> 32^3 analytic: err %.3e\n", e);
+    CHECK(e < 1e-2, "cube periodic double err %g", e);
+    e = cube_periodic<float>(32);
+    std::printf("LaplCube<float, periodic^3> 32^3 analytic: err %.3e\n", e);
+    CHECK(e < 1e-2, "cube periodic float err %g", e);
     e = rect_biquadratic<double>(31);
     std::printf("LaplRect<double> 31^2 biquadratic: err %.3e\n", e);
     CHECK(e < 1e-14, "rect double err %g", e);

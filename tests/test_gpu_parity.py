@@ -4,7 +4,9 @@ reference's golden vectors.
 Tolerances (stated per test):
   * FP64 plans: ||u_gpu - u_cpu||_2 / ||u_cpu||_2 <= 1e-12 (the north-star
     contract "plain GPU FP64 must match the CPU path to <= 1e-12 relative");
-  * FP32 plans: relative L2 error vs the FP32 oracle <= 2e-5 (float rounding);
+  * FP32 plans: relative L2 error vs the FP64 and FP32 oracles <= 2e-6 (float
+    rounding; the reference's own float path is ~4e-7 from the FP64 answer),
+    and <= 2e-5 vs the reference's float golden vectors;
   * FP32+refine plans: relative residual ||f - A u|| / ||f|| <= 1e-10 computed
     in FP64 on the host, and error vs the FP64 oracle <= 1e-9.
 Every GPU step prints its case first (flushed) so a failing box names it.
@@ -73,9 +75,10 @@ def test_cube_f32_vs_oracle(fdm, shape):
     s = fdm.LaplCube(*g, nx, ny, nz, precision="f32")
     ans = np.zeros_like(rhs32)
     s.solve(ans, rhs32)
-    # FP32 tolerance: both are ~1e-6 away from the exact FP64 answer
-    assert rel2(ans, ref64) <= 2e-5
-    assert rel2(ans, ref) <= 2e-5
+    # FP32 tolerance: the reference's float path is ~4e-7 away from the FP64
+    # answer at 127^3; the GPU (float data, double tridiagonal elimination) ~6e-7
+    assert rel2(ans, ref64) <= 2e-6
+    assert rel2(ans, ref) <= 2e-6
 
 
 @pytest.mark.parametrize("n", [15, 31])

@@ -156,6 +156,25 @@ def test_cube_periodic_vs_golden(tag):
     assert rel(ans, GOLD[f"cubep_{tag}_16_ans"].astype(np.float64)) < (1e-13 if tag == "d" else 2e-5)
 
 
+@pytest.mark.parametrize("n", [2, 4])
+def test_cube_periodic_dense_pseudoinverse(n):
+    """Periodic LaplCube = zero-mean least-squares solution of the singular
+    periodic 7-point system (the zeroed zero mode, src/lapl_cube.cpp:82-84)."""
+    h = 2 * np.pi / n
+    f = O.field(5, (n, n, n)) + 0.3
+    u = O.lapl_cube(f, h, h, h, n * h, n * h, n * h, periodic=True)
+    N = n ** 3
+    A = np.zeros((N, N))
+    for c in range(N):
+        e = np.zeros(N)
+        e[c] = 1
+        v = e.reshape(n, n, n)
+        A[:, c] = sum((np.roll(v, 1, a) - 2 * v + np.roll(v, -1, a)) / h ** 2 for a in range(3)).ravel()
+    ref = np.linalg.lstsq(A, f.ravel(), rcond=None)[0]
+    ref -= ref.mean()
+    assert rel(u.ravel(), ref) < 1e-13
+
+
 @pytest.mark.parametrize("n", [7, 31, 63])
 def test_cube_manufactured(n):
     u, f, h, l = O.manufactured_cube(n, seed=1)
