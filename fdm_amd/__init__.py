@@ -31,9 +31,13 @@ ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_ADMISSION = 
 
 # every entry point include/fdmgpu.h declares
 EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_rect_set_scales", "fdmgpu_set_refine",
-           "fdmgpu_solve_host", "fdmgpu_solve_device", "fdmgpu_solve_batch_device", "fdmgpu_get_stats",
-           "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_destroy",
-           "fdmgpu_strerror", "fdmgpu_version")
+           "fdmgpu_solve_host", "fdmgpu_solve_device", "fdmgpu_solve_batch_device", "fdmgpu_solve_table_device",
+           "fdmgpu_plan_variants", "fdmgpu_plan_set_variant", "fdmgpu_plan_get_variant", "fdmgpu_get_stats",
+           "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_group_create",
+           "fdmgpu_group_solve", "fdmgpu_group_result", "fdmgpu_group_destroy", "fdmgpu_server_create",
+           "fdmgpu_server_submit", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
+           "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_destroy", "fdmgpu_strerror",
+           "fdmgpu_version")
 
 # kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
 K_XY_FWD, K_XY_INV, K_X_TILE, K_Y_TILE, K_ZCR, K_ROWCR, K_RESIDUAL, K_OTHER = range(8)
@@ -51,6 +55,29 @@ class FdmGpuError(RuntimeError):
 
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("rel_residual", C.c_double), ("max_rel_residual", C.c_double)]
+
+
+class Variant(C.Structure):
+    """fdmgpu_variant: kernel variant (tile shapes) of a plan."""
+    _fields_ = [("plane", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("zcr_gmul", C.c_int)]
+
+    def astuple(self):
+        return (self.plane, self.tile_x, self.tile_y, self.zcr_gmul)
+
+
+class Problem(C.Structure):
+    """fdmgpu_problem: one solve of a heterogeneous batch or serving request."""
+    _fields_ = [("kind", C.c_int), ("precision", C.c_int), ("periodic", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("d_ans", C.c_void_p), ("d_rhs", C.c_void_p)]
+
+
+class Result(C.Structure):
+    """fdmgpu_result: per-problem outcome (state 0 queued, 1 running, 2 done)."""
+    _fields_ = [("state", C.c_int), ("status", C.c_int), ("iterations", C.c_int),
+                ("rel_residual", C.c_double), ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
 def build(force: bool = True) -> None:
@@ -83,6 +110,23 @@ def _lib_handle():
         L.fdmgpu_plan_footprint.argtypes = [vp, i, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
         L.fdmgpu_set_timing.argtypes = [vp, i]
         L.fdmgpu_get_timing.argtypes = [vp, i, C.POINTER(d), C.POINTER(i)]
+        L.fdmgpu_solve_table_device.argtypes = [vp, i, vp, vp, vp]
+        L.fdmgpu_plan_variants.argtypes = [vp, i, C.POINTER(Variant), C.POINTER(i)]
+        L.fdmgpu_plan_set_variant.argtypes = [vp, C.POINTER(Variant)]
+        L.fdmgpu_plan_get_variant.argtypes = [vp, C.POINTER(Variant)]
+        ull = C.c_ulonglong
+        L.fdmgpu_group_create.argtypes = [C.POINTER(vp), i, C.POINTER(Problem), i, i]
+        L.fdmgpu_group_solve.argtypes = [vp, vp]
+        L.fdmgpu_group_result.argtypes = [vp, i, C.POINTER(Result)]
+        L.fdmgpu_group_destroy.argtypes = [vp]
+        L.fdmgpu_server_create.argtypes = [C.POINTER(vp), ull, i, i]
+        L.fdmgpu_server_submit.argtypes = [vp, C.POINTER(Problem), C.POINTER(ull)]
+        L.fdmgpu_server_poll.argtypes = [vp, ull, C.POINTER(Result)]
+        L.fdmgpu_server_wait.argtypes = [vp, ull, C.POINTER(Result)]
+        L.fdmgpu_server_drain.argtypes = [vp]
+        L.fdmgpu_server_forget.argtypes = [vp, ull]
+        L.fdmgpu_server_destroy.argtypes = [vp]
+        L.fdmgpu_tune.argtypes = [i, C.POINTER(Problem), i, C.c_char_p]
         L.fdmgpu_destroy.argtypes = [vp]
         L.fdmgpu_strerror.argtypes = [i]
         L.fdmgpu_strerror.restype = C.c_char_p
@@ -203,6 +247,32 @@ class _Plan:
             if n.value:
                 out[name] = (ms.value, n.value)
         return out
+
+    def variants(self):
+        """Kernel variants valid for this plan (tuples plane, tile_x, tile_y, zcr_gmul)."""
+        L = _lib_handle()
+        n = C.c_int()
+        _check(L.fdmgpu_plan_variants(self._h, 0, None, C.byref(n)), "plan_variants")
+        arr = (Variant * max(1, n.value))()
+        _check(L.fdmgpu_plan_variants(self._h, n.value, arr, C.byref(n)), "plan_variants")
+        return [arr[k].astuple() for k in range(n.value)]
+
+    @property
+    def variant(self):
+        v = Variant()
+        _check(_lib_handle().fdmgpu_plan_get_variant(self._h, C.byref(v)), "plan_get_variant")
+        return v.astuple()
+
+    @variant.setter
+    def variant(self, t):
+        v = Variant(*t)
+        _check(_lib_handle().fdmgpu_plan_set_variant(self._h, C.byref(v)), "plan_set_variant")
+
+    def solve_table(self, ans_tab, rhs_tab, count: int, stream=None):
+        """Batch of `count` problems at device pointers ans_tab[p] / rhs_tab[p]
+        (device arrays of uint64 pointers, e.g. hip.DeviceArray)."""
+        _check(_lib_handle().fdmgpu_solve_table_device(self._h, count, ans_tab.ptr, rhs_tab.ptr,
+                                                       _stream_handle(stream)), f"{self._kind}.solve_table")
 
     def set_refine(self, rel_tol: float = 1e-10, max_iter: int = 4):
         _check(_lib_handle().fdmgpu_set_refine(self._h, rel_tol, max_iter), "set_refine")

@@ -24,62 +24,11 @@
 #include <new>
 #include <vector>
 
-#include "../../include/fdmgpu.h"
 #include "kernels.hpp"
+#include "plan.hpp"
 #include "refine.hpp"
 
 using namespace fdmgpu;
-
-struct fdmgpu_plan_t {
-    int kind = 3;          // 3 = LaplCube, 2 = LaplRect
-    int precision = FDMGPU_F64;
-    int periodic = 0;
-    int kfwd = 0, kinv = 0;             // pencil transform kinds (kDst, or kRdftFwd/kRdftInv if periodic)
-    int nx = 0, ny = 0, nz = 1;
-    int logx = 0, logy = 0, logz = 0;   // log2 of the transform lengths
-    double dx = 1, dy = 1, dz = 1, lx = 1, ly = 1, lz = 1;
-    double hx_op = 1, hz_op = 1;        // spacings of the inverted operator (eigenvalue aliasing)
-    int device = 0;
-    size_t elem = 8;                    // bytes per element at the API
-    size_t csize = 8;                   // bytes per compute element
-    void* tab_x = nullptr;              // DST tables (compute type), x axis
-    void* tab_y = nullptr;              // y axis (== tab_x when the lengths match)
-    void* lmx = nullptr;                // eigenvalues per mode (compute type)
-    void* lmy = nullptr;
-    // LaplRect scales, per column (index 0..nx-1 = reference index 1..nx)
-    std::vector<double> h_sc, h_Ls, h_Us;
-    void* d_sc = nullptr;               // compute type
-    void* d_Ls = nullptr;
-    void* d_Us = nullptr;
-    double* d_sc64 = nullptr;           // FP64 copies for the residual
-    double* d_Ls64 = nullptr;
-    double* d_Us64 = nullptr;
-    bool scaled = false;
-    // refinement
-    double rtol = 1e-10;
-    int max_iter = 4;
-    float* r32 = nullptr;               // [cap][elems]
-    double* partial = nullptr;          // [cap][nblk][2]
-    int* done = nullptr;                // [cap]
-    int* iters = nullptr;
-    double* resid = nullptr;
-    int cap = 0;
-    double* fcopy = nullptr;            // f kept when a refine solve runs in place
-    size_t fcopy_bytes = 0;
-    // host staging
-    void* stage = nullptr;
-    size_t stage_bytes = 0;
-    hipStream_t own = nullptr;
-    hipStream_t last_stream = nullptr;
-    int last_count = 0;
-    // per-kernel timing (fdmgpu_set_timing)
-    bool timing = false;
-    std::vector<hipEvent_t> evs;        // 2 per recorded launch
-    std::vector<int> rec_kernel;
-    size_t nrec = 0;
-    double tot_ms[FDMGPU_NUM_KERNELS] = {};
-    int launches[FDMGPU_NUM_KERNELS] = {};
-};
 
 namespace {
 
@@ -166,30 +115,132 @@ int alloc_tables(fdmgpu_plan_t* P) {
     return FDMGPU_OK;
 }
 
-int ensure_refine(fdmgpu_plan_t* P, int count) {
-    if (P->cap >= count && P->r32) return FDMGPU_OK;
-    hipFree(P->r32);
-    hipFree(P->partial);
-    hipFree(P->done);
-    hipFree(P->iters);
-    hipFree(P->resid);
-    P->r32 = nullptr;
-    P->partial = nullptr;
-    P->done = P->iters = nullptr;
-    P->resid = nullptr;
-    P->cap = 0;
-    const long elems = (long)P->nx * P->ny * P->nz;
-    // partial norms: residual_kernel writes residual_partials() per problem,
-    // plane_resid_dst_kernel one per z-plane
-    const long nblk = std::max<long>(residual_partials(P->nx, P->ny, P->nz), P->nz);
-    HIPCHK(hipMalloc(&P->r32, sizeof(float) * elems * count));
-    HIPCHK(hipMalloc(&P->partial, sizeof(double) * 2 * nblk * count));
-    HIPCHK(hipMalloc(&P->done, sizeof(int) * count));
-    HIPCHK(hipMalloc(&P->iters, sizeof(int) * count));
-    HIPCHK(hipMalloc(&P->resid, sizeof(double) * count));
-    P->cap = count;
+}  // namespace
+
+namespace fdmgpu {
+
+void set_error_text(const char* msg) { std::snprintf(g_errbuf, sizeof g_errbuf, "%s", msg); }
+
+static long partial_blocks(const fdmgpu_plan_t* P) {
+    // residual_kernel writes residual_partials() per problem, plane_resid_dst_kernel one per z-plane
+    return std::max<long>(residual_partials(P->nx, P->ny, P->nz), P->nz);
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t work_bytes_shape(int precision, int nx, int ny, int nz, int count) {
+    if (precision != FDMGPU_F32_REFINE64 || count <= 0) return 0;
+    const long nblk = std::max<long>(residual_partials(nx, ny, nz), nz);
+    const size_t elems = (size_t)nx * ny * nz;
+    return align256(sizeof(float) * elems * count) + align256(sizeof(double) * 2 * nblk * count) +
+           2 * align256(sizeof(int) * count) + align256(sizeof(double) * count);
+}
+
+size_t work_bytes(const fdmgpu_plan_t* P, int count) {
+    return work_bytes_shape(P->precision, P->nx, P->ny, P->nz, count);
+}
+
+Work work_carve(const fdmgpu_plan_t* P, int count, void* mem) {
+    Work W;
+    if (P->precision != FDMGPU_F32_REFINE64 || count <= 0) return W;
+    char* c = (char*)mem;
+    const long nblk = partial_blocks(P);
+    W.r32 = (float*)c;
+    c += align256(sizeof(float) * plan_elems(P) * count);
+    W.partial = (double*)c;
+    c += align256(sizeof(double) * 2 * nblk * count);
+    W.done = (int*)c;
+    c += align256(sizeof(int) * count);
+    W.iters = (int*)c;
+    c += align256(sizeof(int) * count);
+    W.resid = (double*)c;
+    W.cap = count;
+    return W;
+}
+
+void work_free(Work& W) {
+    hipFree(W.r32);
+    hipFree(W.partial);
+    hipFree(W.done);
+    hipFree(W.iters);
+    hipFree(W.resid);
+    hipFree(W.fcopy);
+    W = Work{};
+}
+
+int work_reserve(const fdmgpu_plan_t* P, Work& W, int count) {
+    if (W.cap >= count && W.r32) return FDMGPU_OK;
+    const double* fcopy = W.fcopy;
+    const size_t fcb = W.fcopy_bytes;
+    hipFree(W.r32);
+    hipFree(W.partial);
+    hipFree(W.done);
+    hipFree(W.iters);
+    hipFree(W.resid);
+    W = Work{};
+    W.fcopy = const_cast<double*>(fcopy);
+    W.fcopy_bytes = fcb;
+    const long elems = plan_elems(P);
+    const long nblk = partial_blocks(P);
+    HIPCHK(hipMalloc(&W.r32, sizeof(float) * elems * count));
+    HIPCHK(hipMalloc(&W.partial, sizeof(double) * 2 * nblk * count));
+    HIPCHK(hipMalloc(&W.done, sizeof(int) * count));
+    HIPCHK(hipMalloc(&W.iters, sizeof(int) * count));
+    HIPCHK(hipMalloc(&W.resid, sizeof(double) * count));
+    W.cap = count;
     return FDMGPU_OK;
 }
+
+bool plane_capable(const fdmgpu_plan_t* P) {
+    if (P->kind != 3 || P->logx != P->logy) return false;
+    return P->precision == FDMGPU_F64 ? plane_fits<double>(P->logx) : plane_fits<float>(P->logx);
+}
+
+std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
+    const bool f64 = P->precision == FDMGPU_F64;
+    auto tiles = [&](int logn) {
+        std::vector<int> t;
+        for (int c : {16, 32, 64})
+            if (f64 ? tile_ok<double>(logn, c) : tile_ok<float>(logn, c)) t.push_back(c);
+        return t;
+    };
+    std::vector<int> gms{1};
+    if (P->kind == 3 && zcr_gmul_ok(P->logz, 2)) gms.push_back(2);
+    std::vector<Variant> out;
+    Variant d;
+    d.plane = plane_capable(P) ? 1 : 0;
+    out.push_back(d);   // the default first
+    for (int gm : gms) {
+        if (plane_capable(P)) {
+            Variant v;
+            v.plane = 1;
+            v.zcr_gmul = gm;
+            out.push_back(v);
+        }
+        for (int tx : (P->kind == 3 ? tiles(P->logx) : std::vector<int>{0}))
+            for (int ty : tiles(P->logy)) {
+                Variant v;
+                v.plane = 0;
+                v.tile_x = tx;
+                v.tile_y = ty;
+                v.zcr_gmul = gm;
+                out.push_back(v);
+            }
+    }
+    return out;
+}
+
+bool variant_valid(const fdmgpu_plan_t* P, const Variant& v) {
+    for (const Variant& c : plan_variants(P))
+        if (c.plane == v.plane && c.tile_x == v.tile_x && c.tile_y == v.tile_y &&
+            (c.zcr_gmul ? c.zcr_gmul : 1) == (v.zcr_gmul ? v.zcr_gmul : 1))
+            return true;
+    return false;
+}
+
+}  // namespace fdmgpu
+
+namespace {
 
 // Records HIP events around one launch when timing is enabled.
 struct Timed {
@@ -228,31 +279,37 @@ int harvest_timing(fdmgpu_plan_t* P) {
 }
 
 // ---- the three stages, templated on compute type T ------------------------
+// Every stage addresses problem p of a batch at ptr + p * elems, or at tab[p]
+// when a device table of per-problem pointers is given (tab != null).
+using Tab = const void* const*;
 // Forward xy transforms: in (Tin) -> out (T).
 template <typename Tin, typename T>
-int stage_fwd(fdmgpu_plan_t* P, const void* in, void* out, int count, hipStream_t s, const int* done) {
+int stage_fwd(fdmgpu_plan_t* P, const void* in, Tab in_tab, void* out, Tab out_tab, int count, hipStream_t s,
+              const int* done) {
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = P->dx * std::sqrt(2. / P->lx), sy = P->dy * std::sqrt(2. / P->ly);
-    if (P->kind == 3 && P->logx == P->logy && plane_fits<T>(P->logx)) {
-        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done, P->kfwd};
+    if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done, P->kfwd, in_tab, out_tab};
         Timed tm(P, FDMGPU_K_XY_FWD, s);
         HIPCHK((launch_plane<Tin, T, T>(P->logx, a)));
         return FDMGPU_OK;
     }
     if (P->kind == 3) {
         AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, false, done,
-                    P->kfwd};
+                    P->kfwd, in_tab, out_tab, P->var.tile_x};
         {
             Timed tm(P, FDMGPU_K_X_TILE, s);
             HIPCHK((launch_xaxis<Tin, T, T>(P->logx, ax)));
         }
-        AxisArgs ay{out, out, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd};
+        AxisArgs ay{out, out, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd, out_tab,
+                    out_tab, P->var.tile_y};
         Timed tm(P, FDMGPU_K_Y_TILE, s);
         HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
         return FDMGPU_OK;
     }
     // rect: y only
-    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd};
+    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd, in_tab, out_tab,
+                P->var.tile_y};
     Timed tm(P, FDMGPU_K_Y_TILE, s);
     HIPCHK((launch_yaxis<Tin, T, T>(P->logy, ay)));
     return FDMGPU_OK;
@@ -260,34 +317,38 @@ int stage_fwd(fdmgpu_plan_t* P, const void* in, void* out, int count, hipStream_
 
 // Inverse xy transforms: in (T) -> out (Tout), optionally accumulating.
 template <typename T, typename Tout>
-int stage_inv(fdmgpu_plan_t* P, void* in, void* out, int count, hipStream_t s, bool accum, const int* done) {
+int stage_inv(fdmgpu_plan_t* P, void* in, Tab in_tab, void* out, Tab out_tab, int count, hipStream_t s, bool accum,
+              const int* done) {
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = std::sqrt(2. / P->lx), sy = std::sqrt(2. / P->ly);
-    if (P->kind == 3 && P->logx == P->logy && plane_fits<T>(P->logx)) {
-        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv};
+    if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv, in_tab, out_tab};
         Timed tm(P, accum ? FDMGPU_K_XY_INV_ACC : FDMGPU_K_XY_INV, s);
         HIPCHK((launch_plane<T, Tout, T>(P->logx, a)));
         return FDMGPU_OK;
     }
     if (P->kind == 3) {
-        AxisArgs ay{in, in, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv};
+        AxisArgs ay{in, in, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv, in_tab,
+                    in_tab, P->var.tile_y};
         {
             Timed tm(P, FDMGPU_K_Y_TILE, s);
             HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
         }
         AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, accum, done,
-                    P->kinv};
+                    P->kinv, in_tab, out_tab, P->var.tile_x};
         Timed tm(P, FDMGPU_K_X_TILE, s);
         HIPCHK((launch_xaxis<T, Tout, T>(P->logx, ax)));
         return FDMGPU_OK;
     }
     if (accum) {   // rect: inverse y into the FP32 buffer, then accumulate through an x pass-free add
-        AxisArgs ay{in, in, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv};
+        AxisArgs ay{in, in, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv, in_tab, in_tab,
+                    P->var.tile_y};
         Timed tm(P, FDMGPU_K_Y_TILE, s);
         HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
         return -1;   // caller adds
     }
-    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv};
+    AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv, in_tab, out_tab,
+                P->var.tile_y};
     Timed tm(P, FDMGPU_K_Y_TILE, s);
     HIPCHK((launch_yaxis<T, Tout, T>(P->logy, ay)));
     return FDMGPU_OK;
@@ -299,39 +360,39 @@ int stage_inv(fdmgpu_plan_t* P, void* in, void* out, int count, hipStream_t s, b
 // float accuracy.  The inner solves of FP32+refine stay FP32 (refinement
 // recovers the accuracy).
 template <typename T>
-int stage_tridiag(fdmgpu_plan_t* P, void* data, int count, hipStream_t s, const int* done) {
+int stage_tridiag(fdmgpu_plan_t* P, void* data, Tab data_tab, int count, hipStream_t s, const int* done) {
     const long elems = (long)P->nx * P->ny * P->nz;
     if (P->kind == 3) {
         const double b = 1.0 / (P->hz_op * P->hz_op);
         const double gamma = P->dz * (double)(1 << P->logz) / P->lz;
         ZcrArgs z{data, (long)P->nx * P->ny, P->nx, P->lmx, P->lmy, b, -2.0 * b,
                   std::fabs(gamma - 1.0) < 4e-16 ? 1.0 : gamma, elems, count, s, done, P->periodic != 0,
-                  P->precision == FDMGPU_F32};
+                  P->precision == FDMGPU_F32, data_tab, P->var.zcr_gmul};
         Timed tm(P, FDMGPU_K_ZCR, s);
         HIPCHK(launch_zcr<T>(P->logz, z));
         return FDMGPU_OK;
     }
     RowCrArgs r{data, P->nx, P->ny, P->lmy, P->d_sc, P->d_Ls, P->d_Us, 1.0 / (P->dx * P->dx), elems, count, s, done,
-                P->precision == FDMGPU_F32};
+                P->precision == FDMGPU_F32, data_tab};
     Timed tm(P, FDMGPU_K_ROWCR, s);
     HIPCHK(launch_rowcr<T>(r));
     return FDMGPU_OK;
 }
 
 template <typename T>
-int solve_plain(fdmgpu_plan_t* P, void* ans, const void* rhs, int count, hipStream_t s) {
-    int rc = stage_fwd<T, T>(P, rhs, ans, count, s, nullptr);
+int solve_plain(fdmgpu_plan_t* P, void* ans, Tab ans_tab, const void* rhs, Tab rhs_tab, int count, hipStream_t s) {
+    int rc = stage_fwd<T, T>(P, rhs, rhs_tab, ans, ans_tab, count, s, nullptr);
     if (rc) return rc;
-    rc = stage_tridiag<T>(P, ans, count, s, nullptr);
+    rc = stage_tridiag<T>(P, ans, ans_tab, count, s, nullptr);
     if (rc) return rc;
-    return stage_inv<T, T>(P, ans, ans, count, s, false, nullptr);
+    return stage_inv<T, T>(P, ans, ans_tab, ans, ans_tab, count, s, false, nullptr);
 }
 
-__global__ void add_f32_to_f64(double* __restrict__ u, const float* __restrict__ c, const long n,
-                               const long bs, const int* __restrict__ done) {
+__global__ void add_f32_to_f64(const BatchPtr<double> u, const float* __restrict__ c, const long n, const long bs,
+                               const int* __restrict__ done) {
     if (done && done[blockIdx.y]) return;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) u[blockIdx.y * bs + i] += (double)c[blockIdx.y * bs + i];
+    if (i < n) u(blockIdx.y)[i] += (double)c[blockIdx.y * bs + i];
 }
 
 // FP32 inner solves + FP64 residual/update/norm.
@@ -341,33 +402,34 @@ __global__ void add_f32_to_f64(double* __restrict__ u, const float* __restrict__
 //   finally:      ||f - A u||/||f|| of the last iterate.
 // A problem whose residual reached rtol is flagged on the device; every later
 // launch skips it, so the host never synchronises inside a solve.
-int solve_refine(fdmgpu_plan_t* P, double* ans, const double* rhs, int count, hipStream_t s) {
-    int rc = ensure_refine(P, count);
+int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const double* rhs, Tab rhs_tab,
+                 int count, hipStream_t s) {
+    int rc = work_reserve(P, W, count);
     if (rc) return rc;
     const long elems = (long)P->nx * P->ny * P->nz;
-    if ((const void*)ans == (const void*)rhs) {   // f is needed by every residual: keep a copy
+    if (!ans_tab && (const void*)ans == (const void*)rhs) {   // f is needed by every residual: keep a copy
         const size_t bytes = sizeof(double) * elems * count;
-        if (P->fcopy_bytes < bytes) {
-            hipFree(P->fcopy);
-            P->fcopy = nullptr;
-            P->fcopy_bytes = 0;
-            HIPCHK(hipMalloc(&P->fcopy, bytes));
-            P->fcopy_bytes = bytes;
+        if (W.fcopy_bytes < bytes) {
+            hipFree(W.fcopy);
+            W.fcopy = nullptr;
+            W.fcopy_bytes = 0;
+            HIPCHK(hipMalloc(&W.fcopy, bytes));
+            W.fcopy_bytes = bytes;
         }
         Timed tm(P, FDMGPU_K_OTHER, s);
-        HIPCHK(hipMemcpyAsync(P->fcopy, rhs, bytes, hipMemcpyDeviceToDevice, s));
-        rhs = P->fcopy;
+        HIPCHK(hipMemcpyAsync(W.fcopy, rhs, bytes, hipMemcpyDeviceToDevice, s));
+        rhs = W.fcopy;
     }
     {
         Timed tm(P, FDMGPU_K_OTHER, s);
-        HIPCHK(launch_fill_int(P->done, count, 0, s));
-        HIPCHK(launch_fill_int(P->iters, count, 0, s));
+        HIPCHK(launch_fill_int(W.done, count, 0, s));
+        HIPCHK(launch_fill_int(W.iters, count, 0, s));
     }
     ResidualArgs ra{};
     ra.f = rhs;
     ra.u = ans;
-    ra.r32 = P->r32;
-    ra.partial = P->partial;
+    ra.r32 = W.r32;
+    ra.partial = W.partial;
     ra.nx = P->nx;
     ra.ny = P->ny;
     ra.nz = P->nz;
@@ -381,11 +443,13 @@ int solve_refine(fdmgpu_plan_t* P, double* ans, const double* rhs, int count, hi
     ra.bs = elems;
     ra.batch = count;
     ra.tol = P->rtol;
-    ra.done = P->done;
-    ra.iters = P->iters;
-    ra.resid = P->resid;
+    ra.done = W.done;
+    ra.iters = W.iters;
+    ra.resid = W.resid;
     ra.stream = s;
-    const bool fused = P->kind == 3 && P->logx == P->logy && plane_fits<float>(P->logx);
+    ra.f_tab = rhs_tab;
+    ra.u_tab = ans_tab;
+    const bool fused = plane_capable(P) && P->var.plane;
 
     // Fused path (cubes whose planes fit LDS): the forward DST of solve i >= 2
     // computes r = f - A u itself; convergence of the iterate after solve i is
@@ -393,55 +457,69 @@ int solve_refine(fdmgpu_plan_t* P, double* ans, const double* rhs, int count, hi
     // pass writes r32 (FP32) for the next solve and certifies at the same time.
     for (int it = 1; it <= P->max_iter; ++it) {
         if (it == 1) {
-            rc = stage_fwd<double, float>(P, rhs, P->r32, count, s, P->done);
+            rc = stage_fwd<double, float>(P, rhs, rhs_tab, W.r32, nullptr, count, s, W.done);
         } else if (fused) {
-            PlaneResidArgs pa{rhs, ans, P->r32, P->partial, elems, P->nz, count, ra.ix, ra.iy, ra.iz,
-                              P->dx * std::sqrt(2. / P->lx), P->dy * std::sqrt(2. / P->ly), P->tab_x, s, P->done};
+            PlaneResidArgs pa{rhs, ans, W.r32, W.partial, elems, P->nz, count, ra.ix, ra.iy, ra.iz,
+                              P->dx * std::sqrt(2. / P->lx), P->dy * std::sqrt(2. / P->ly), P->tab_x, s, W.done,
+                              rhs_tab, ans_tab};
             {
                 Timed tm(P, FDMGPU_K_XY_FWD_RESID, s);
                 HIPCHK(launch_plane_resid<double>(P->logx, pa));
             }
             if (it == 2) {   // the iterate after solve 1 was not certified
                 Timed tm(P, FDMGPU_K_RESIDUAL, s);
-                HIPCHK(launch_check(P->partial, P->nz, P->rtol, P->done, P->iters, P->resid, 1, count, s));
+                HIPCHK(launch_check(W.partial, P->nz, P->rtol, W.done, W.iters, W.resid, 1, count, s));
             }
             rc = 0;
         } else {
-            rc = stage_fwd<float, float>(P, P->r32, P->r32, count, s, P->done);
+            rc = stage_fwd<float, float>(P, W.r32, nullptr, W.r32, nullptr, count, s, W.done);
         }
         if (rc) return rc;
-        rc = stage_tridiag<float>(P, P->r32, count, s, P->done);
+        rc = stage_tridiag<float>(P, W.r32, nullptr, count, s, W.done);
         if (rc) return rc;
-        rc = stage_inv<float, double>(P, P->r32, ans, count, s, it > 1, P->done);
+        rc = stage_inv<float, double>(P, W.r32, nullptr, ans, ans_tab, count, s, it > 1, W.done);
         if (rc == -1) {   // rect accumulate
             dim3 g((unsigned)((elems + 255) / 256), count);
             Timed tm(P, FDMGPU_K_OTHER, s);
-            hipLaunchKernelGGL(add_f32_to_f64, g, dim3(256), 0, s, ans, (const float*)P->r32, elems, elems,
-                               (const int*)P->done);
+            hipLaunchKernelGGL(add_f32_to_f64, g, dim3(256), 0, s, BatchPtr<double>{ans, ans_tab, elems},
+                               (const float*)W.r32, elems, elems, (const int*)W.done);
             HIPCHK(hipGetLastError());
             rc = 0;
         }
         if (rc) return rc;
         if (fused && it == 1 && P->max_iter > 1) continue;   // solve 2's fused pass checks it
         ra.it = it;
-        ra.r32 = (fused || it == P->max_iter) ? nullptr : P->r32;
+        ra.r32 = (fused || it == P->max_iter) ? nullptr : W.r32;
         Timed tm(P, FDMGPU_K_RESIDUAL, s);
         HIPCHK(launch_residual(ra));
     }
     return FDMGPU_OK;
 }
 
-int solve_device_impl(fdmgpu_plan_t* P, void* ans, const void* rhs, int count, hipStream_t s) {
-    HIPCHK(hipSetDevice(P->device));
+int solve_device_impl(fdmgpu_plan_t* P, void* ans, Tab ans_tab, const void* rhs, Tab rhs_tab, int count,
+                      hipStream_t s) {
     P->last_stream = s;
     P->last_count = count;
+    return plan_solve(P, P->w, ans, ans_tab, rhs, rhs_tab, count, s);
+}
+
+}  // namespace
+
+int fdmgpu::plan_solve(fdmgpu_plan_t* P, Work& W, void* ans, Tab ans_tab, const void* rhs, Tab rhs_tab, int count,
+                       hipStream_t s) {
+    HIPCHK(hipSetDevice(P->device));
     if (count <= 0) return FDMGPU_OK;
+    // the launchers check hipGetLastError(); drop a stale error of an earlier,
+    // unrelated runtime call on this thread so it is not reported as ours
+    (void)hipGetLastError();
     switch (P->precision) {
-        case FDMGPU_F32: return solve_plain<float>(P, ans, rhs, count, s);
-        case FDMGPU_F64: return solve_plain<double>(P, ans, rhs, count, s);
-        default: return solve_refine(P, (double*)ans, (const double*)rhs, count, s);
+        case FDMGPU_F32: return solve_plain<float>(P, ans, ans_tab, rhs, rhs_tab, count, s);
+        case FDMGPU_F64: return solve_plain<double>(P, ans, ans_tab, rhs, rhs_tab, count, s);
+        default: return solve_refine(P, W, (double*)ans, ans_tab, (const double*)rhs, rhs_tab, count, s);
     }
 }
+
+namespace {
 
 int common_init(fdmgpu_plan_t* P, int precision) {
     P->precision = precision;
@@ -469,13 +547,8 @@ void free_plan(fdmgpu_plan_t* P) {
     hipFree(P->d_sc64);
     hipFree(P->d_Ls64);
     hipFree(P->d_Us64);
-    hipFree(P->r32);
-    hipFree(P->partial);
-    hipFree(P->done);
-    hipFree(P->iters);
-    hipFree(P->resid);
+    work_free(P->w);
     hipFree(P->stage);
-    hipFree(P->fcopy);
     for (hipEvent_t e : P->evs) (void)hipEventDestroy(e);
     if (P->own) hipStreamDestroy(P->own);
     delete P;
@@ -595,6 +668,8 @@ int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double
         free_plan(P);
         return rc;
     }
+    P->var.plane = plane_capable(P) ? 1 : 0;   // the default variant
+    apply_tuned(P);
     *plan = P;
     return FDMGPU_OK;
 }
@@ -634,6 +709,8 @@ int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double
         free_plan(P);
         return rc;
     }
+    P->var.plane = plane_capable(P) ? 1 : 0;   // the default variant
+    apply_tuned(P);
     *plan = P;
     return FDMGPU_OK;
 }
@@ -654,13 +731,42 @@ int fdmgpu_set_refine(fdmgpu_plan plan, double rel_tol, int max_iter) {
 
 int fdmgpu_solve_device(fdmgpu_plan plan, void* d_ans, const void* d_rhs, void* stream) {
     if (!plan || !d_ans || !d_rhs) return FDMGPU_ERR_INVALID_ARG;
-    return solve_device_impl(plan, d_ans, d_rhs, 1, (hipStream_t)stream);
+    return solve_device_impl(plan, d_ans, nullptr, d_rhs, nullptr, 1, (hipStream_t)stream);
 }
 
 int fdmgpu_solve_batch_device(fdmgpu_plan plan, int count, void* d_ans, const void* d_rhs, void* stream) {
     if (!plan || count < 0 || (count > 0 && (!d_ans || !d_rhs))) return FDMGPU_ERR_INVALID_ARG;
     if (count > 65535) return FDMGPU_ERR_UNSUPPORTED;
-    return solve_device_impl(plan, d_ans, d_rhs, count, (hipStream_t)stream);
+    return solve_device_impl(plan, d_ans, nullptr, d_rhs, nullptr, count, (hipStream_t)stream);
+}
+
+int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_tab, const void* const* d_rhs_tab,
+                              void* stream) {
+    if (!plan || count < 0 || (count > 0 && (!d_ans_tab || !d_rhs_tab))) return FDMGPU_ERR_INVALID_ARG;
+    if (count > 65535) return FDMGPU_ERR_UNSUPPORTED;
+    return solve_device_impl(plan, nullptr, (Tab)d_ans_tab, nullptr, d_rhs_tab, count, (hipStream_t)stream);
+}
+
+int fdmgpu_plan_variants(fdmgpu_plan plan, int max, fdmgpu_variant* out, int* count) {
+    if (!plan || max < 0 || (max > 0 && !out)) return FDMGPU_ERR_INVALID_ARG;
+    const std::vector<Variant> v = plan_variants(plan);
+    for (int i = 0; i < max && i < (int)v.size(); i++) out[i] = {v[i].plane, v[i].tile_x, v[i].tile_y, v[i].zcr_gmul};
+    if (count) *count = (int)v.size();
+    return FDMGPU_OK;
+}
+
+int fdmgpu_plan_set_variant(fdmgpu_plan plan, const fdmgpu_variant* v) {
+    if (!plan || !v) return FDMGPU_ERR_INVALID_ARG;
+    const Variant w{v->plane, v->tile_x, v->tile_y, v->zcr_gmul};
+    if (!variant_valid(plan, w)) return FDMGPU_ERR_INVALID_ARG;
+    plan->var = w;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_plan_get_variant(fdmgpu_plan plan, fdmgpu_variant* v) {
+    if (!plan || !v) return FDMGPU_ERR_INVALID_ARG;
+    *v = {plan->var.plane, plan->var.tile_x, plan->var.tile_y, plan->var.zcr_gmul};
+    return FDMGPU_OK;
 }
 
 int fdmgpu_solve_host(fdmgpu_plan plan, void* ans, const void* rhs) {
@@ -679,7 +785,7 @@ int fdmgpu_solve_host(fdmgpu_plan plan, void* ans, const void* rhs) {
     char* d_rhs = (char*)plan->stage;
     char* d_ans = plan->precision == FDMGPU_F32_REFINE64 ? d_rhs + bytes : d_rhs;
     HIPCHK(hipMemcpyAsync(d_rhs, rhs, bytes, hipMemcpyHostToDevice, plan->own));
-    int rc = solve_device_impl(plan, d_ans, d_rhs, 1, plan->own);
+    int rc = solve_device_impl(plan, d_ans, nullptr, d_rhs, nullptr, 1, plan->own);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(ans, d_ans, bytes, hipMemcpyDeviceToHost, plan->own));
     HIPCHK(hipStreamSynchronize(plan->own));
@@ -703,8 +809,8 @@ int fdmgpu_get_stats(fdmgpu_plan plan, fdmgpu_stats* st) {
         const int c = plan->last_count;
         std::vector<int> it(c);
         std::vector<double> r(c);
-        HIPCHK(hipMemcpy(it.data(), plan->iters, sizeof(int) * c, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(r.data(), plan->resid, sizeof(double) * c, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(it.data(), plan->w.iters, sizeof(int) * c, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(r.data(), plan->w.resid, sizeof(double) * c, hipMemcpyDeviceToHost));
         st->iterations = *std::max_element(it.begin(), it.end());
         st->rel_residual = r[c - 1];
         st->max_rel_residual = *std::max_element(r.begin(), r.end());
@@ -715,15 +821,11 @@ int fdmgpu_get_stats(fdmgpu_plan plan, fdmgpu_stats* st) {
 int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resident,
                           unsigned long long* per_solve) {
     if (!plan || count < 0) return FDMGPU_ERR_INVALID_ARG;
-    const unsigned long long elems = (unsigned long long)plan->nx * plan->ny * plan->nz;
     const unsigned long long tabs = 3ull * ((1ull << plan->logx) + (1ull << plan->logy)) * plan->csize +
                                     (unsigned long long)(plan->nx + plan->ny) * plan->csize;
     if (resident) *resident = tabs;
     if (per_solve) {
-        unsigned long long s = 0;
-        if (plan->precision == FDMGPU_F32_REFINE64)
-            s = count * (elems * 4 + 16ull * residual_partials(plan->nx, plan->ny, plan->nz) + 16);
-        *per_solve = s;
+        *per_solve = work_bytes(plan, count);
     }
     return FDMGPU_OK;
 }
@@ -753,8 +855,10 @@ int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launc
 
 int fdmgpu_destroy(fdmgpu_plan plan) {
     if (!plan) return FDMGPU_ERR_INVALID_ARG;
-    hipSetDevice(plan->device);
-    if (plan->last_stream || plan->own) hipStreamSynchronize(plan->last_stream);
+    (void)hipSetDevice(plan->device);
+    // the stream of the last solve may be gone already: wait for the device
+    // instead, so no kernel still reads the tables or scratch freed below
+    (void)hipDeviceSynchronize();
     free_plan(plan);
     return FDMGPU_OK;
 }

@@ -5,6 +5,19 @@
 
 namespace fdmgpu {
 
+// Where problem p of a batch lives: base + p*bs (problems back to back), or
+// tab[p] when a device table of per-problem pointers is given (problems in
+// separate buffers, e.g. the requests of one serving batch).
+template <typename T>
+struct BatchPtr {
+    T* base;
+    const void* const* tab;
+    long bs;
+    __host__ __device__ T* operator()(int p) const {
+        return tab ? (T*)tab[p] : base + (long)p * bs;
+    }
+};
+
 // Pencil transform kinds (see pencil_pairs in dst_device.hpp): DST-I on
 // Dirichlet axes, half-complex real DFT forward / inverse on periodic axes.
 enum : int { kDst = 0, kRdftFwd = 1, kRdftInv = 2 };
@@ -20,6 +33,8 @@ struct PlaneArgs {
     bool accum;           // out += result (refinement update) instead of out = result
     const int* done;      // per batch item: skip when non-zero (may be null)
     int kind;             // kDst (0), kRdftFwd (1), kRdftInv (2); periodic kinds need Tin == Tout == T
+    const void* const* in_tab;    // per-problem pointers (device) instead of in + p*bs_in, or null
+    const void* const* out_tab;
 };
 
 struct AxisArgs {
@@ -35,6 +50,9 @@ struct AxisArgs {
     bool accum;           // x kernel only
     const int* done;
     int kind;             // as PlaneArgs::kind
+    const void* const* in_tab;
+    const void* const* out_tab;
+    int tile;             // pencils per block (16/32/64; 0 = default_tile)
 };
 
 struct ZcrArgs {
@@ -50,6 +68,8 @@ struct ZcrArgs {
     const int* done;
     bool periodic;        // circulant system of 2^q rows instead of 2^q - 1 rows
     bool wide;            // float data, double arithmetic
+    const void* const* data_tab;
+    int gmul;             // lanes-per-system multiplier (1 or 2; 0 = 1)
 };
 
 struct RowCrArgs {
@@ -65,6 +85,7 @@ struct RowCrArgs {
     hipStream_t stream;
     const int* done;
     bool wide;            // float data, double arithmetic
+    const void* const* data_tab;
 };
 
 // Forward plane DST of r = f - A u (FP64 residual fused into the load) to FP32,
@@ -81,6 +102,8 @@ struct PlaneResidArgs {
     const void* tables;   // FP32 DST tables
     hipStream_t stream;
     const int* done;
+    const void* const* f_tab;   // per-problem pointers or null
+    const void* const* u_tab;
 };
 
 template <typename Tin, typename Tout, typename T>
@@ -95,6 +118,11 @@ template <typename Tin, typename Tout, typename T>
 hipError_t launch_yaxis(int logn, const AxisArgs& a);
 template <typename T>
 hipError_t launch_zcr(int q, const ZcrArgs& a);
+template <typename T>
+bool tile_ok(int logn, int tile);
+template <typename T>
+int default_tile(int logn);
+bool zcr_gmul_ok(int q, int gm);
 template <typename T>
 hipError_t launch_rowcr(const RowCrArgs& a);
 

@@ -1,0 +1,272 @@
+// Plane kernels (gfx950): transforms along x and y of whole z-planes held in
+// LDS, and the refinement's forward transform with the FP64 residual fused
+// into its load.
+//
+// Data layout in HBM: a field is dense [nz][ny][nx] (x fastest), interior
+// points only, exactly the array LaplCube::solve receives (reference
+// src/lapl_cube.h:80, src/tensor.h:192-205).  Batches of equal-shape problems
+// are back to back (batch stride = nx*ny*nz elements) or addressed through a
+// per-problem pointer table.  The transform is the DST-I on Dirichlet axes and
+// the half-complex real DFT pair on periodic ones (template parameter KIND).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "dst_device.hpp"
+#include "kernel_util.hpp"
+#include "kernels.hpp"
+
+namespace fdmgpu {
+
+// ---------------------------------------------------------------------------
+// plane kernel
+// ---------------------------------------------------------------------------
+template <int LOGN>
+struct PlaneCfg {
+    using S = DstShape<LOGN>;
+    static constexpr int N = S::N;
+    static constexpr int P = N + S::PAD;
+    static constexpr int WORK = (N / 2) * S::N1;
+    static constexpr int NT = WORK < 64 ? 64 : WORK;
+    template <typename T>
+    static constexpr size_t lds_bytes() {
+        return sizeof(T) * (size_t)N * P;
+    }
+};
+
+// KIND (dst_device.hpp): kDst for Dirichlet planes of (N-1)^2 points,
+// kRdftFwd / kRdftInv for periodic planes of N^2 points.
+template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND>
+__global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_dst_kernel(
+    const BatchPtr<const Tin> in, const BatchPtr<Tout> out, const T sx, const T sy, const T* __restrict__ tables,
+    const int* __restrict__ done) {
+    using C = PlaneCfg<LOGN>;
+    using S = DstShape<LOGN>;
+    using IO = PencilIO<KIND, LOGN>;
+    using LX = PlaneLayout<C::P, S::N1, true>;
+    using LY = PlaneLayout<C::P, S::N1, false>;
+    if (done && done[blockIdx.y]) return;
+    constexpr int N = C::N, n = IO::n, o = IO::off, NT = C::NT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    const int tid = threadIdx.x;
+    const long plane_off = (long)blockIdx.x * n * n;
+    const Tin* __restrict__ src = in(blockIdx.y) + plane_off;
+    Tout* __restrict__ dst = out(blockIdx.y) + plane_off;
+
+    DstLane<T, LOGN> L;
+    L.init(tables, tid & (S::N1 - 1));
+    if constexpr (KIND == kDst) {
+        for (int i = tid; i < N; i += NT) {
+            lds[LX::at(0, i)] = T(0);   // dummy row 0
+            lds[LX::at(i, 0)] = T(0);   // dummy column 0
+        }
+    }
+    for (int idx = tid; idx < n * n; idx += NT) {
+        const int y = idx / n, x = idx - y * n;
+        lds[LX::at(y + o, x + o)] = (T)src[idx];
+    }
+    __syncthreads();
+    pencil_pairs<KIND, T, LOGN, N / 2, LX, NT>(lds, L, sx, tid);   // pencils = rows    (x)
+    __syncthreads();
+    pencil_pairs<KIND, T, LOGN, N / 2, LY, NT>(lds, L, sy, tid);   // pencils = columns (y)
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += NT) {
+        const int y = idx / n, x = idx - y * n;
+        const T v = lds[LX::at(IO::out(y), IO::out(x))];
+        if constexpr (ACCUM) dst[idx] += (Tout)v;
+        else dst[idx] = (Tout)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Refinement: forward DST of the FP64 residual r = f - A u, fused into the
+// plane load.  A is the 7-point operator the solve inverts (1/h^2 per axis:
+// ix, iy, iz).  Each block also writes (sum r^2, sum f^2) of its plane to
+// partial[p][z] for the convergence check.  Planes are dealt to XCDs in
+// contiguous z-slabs (workgroups go round-robin over the 8 XCDs), so a
+// block's z-1 / z+1 neighbour planes are read by the same XCD close in time
+// and come from its L2.
+// ---------------------------------------------------------------------------
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_slab_plane(int bid, int nplanes) {
+    const int per = (nplanes + kXcds - 1) / kXcds;
+    return (bid % kXcds) * per + bid / kXcds;
+}
+
+template <typename Tu>
+__device__ __forceinline__ double residual_at(const double* __restrict__ f, const Tu* __restrict__ u, const long c,
+                                              const int x, const int y, const int z, const int nx, const int ny,
+                                              const int nz, const double ix, const double iy, const double iz,
+                                              double& fv) {
+    const long pl = (long)nx * ny;
+    const double uc = (double)u[c];
+    const double xm = x > 0 ? (double)u[c - 1] : 0.0, xp = x < nx - 1 ? (double)u[c + 1] : 0.0;
+    const double ym = y > 0 ? (double)u[c - nx] : 0.0, yp = y < ny - 1 ? (double)u[c + nx] : 0.0;
+    const double zm = z > 0 ? (double)u[c - pl] : 0.0, zp = z < nz - 1 ? (double)u[c + pl] : 0.0;
+    fv = f[c];
+    return fv - ((xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy + (zm - 2.0 * uc + zp) * iz);
+}
+
+template <int NT>
+__device__ __forceinline__ void block_sum2_to(double rr, double ff, double* __restrict__ out,
+                                              double* __restrict__ red) {
+    // wave reduction, then the waves' partials in fixed order
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        rr += __shfl_xor(rr, d, 64);
+        ff += __shfl_xor(ff, d, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = rr;
+        red[2 * w + 1] = ff;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < (NT + 63) / 64; ++i) {
+            a += red[2 * i];
+            b += red[2 * i + 1];
+        }
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+template <typename Tu, int LOGN>
+__global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_resid_dst_kernel(
+    const BatchPtr<const double> f, const BatchPtr<const Tu> u, float* __restrict__ out,
+    double* __restrict__ partial, const long bs, const int nplanes, const double ix, const double iy,
+    const double iz, const float sx, const float sy, const float* __restrict__ tables,
+    const int* __restrict__ done) {
+    using C = PlaneCfg<LOGN>;
+    using S = DstShape<LOGN>;
+    using LX = PlaneLayout<C::P, S::N1, true>;
+    using LY = PlaneLayout<C::P, S::N1, false>;
+    constexpr int N = C::N, n = N - 1, NT = C::NT;
+    const int p = blockIdx.y;
+    const int z = xcd_slab_plane(blockIdx.x, nplanes);
+    if (z >= nplanes || (done && done[p])) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float* lds = reinterpret_cast<float*>(smem_raw);
+    double* red = reinterpret_cast<double*>(lds + N * C::P + 4);   // 2 doubles per wave
+    const int tid = threadIdx.x;
+
+    DstLane<float, LOGN> L;
+    L.init(tables, tid & (S::N1 - 1));
+    for (int i = tid; i < N; i += NT) {
+        lds[LX::at(0, i)] = 0.f;
+        lds[LX::at(i, 0)] = 0.f;
+    }
+    const double* __restrict__ fp = f(p);
+    const Tu* __restrict__ up = u(p);
+    double rr = 0.0, ff = 0.0;
+    for (int idx = tid; idx < n * n; idx += NT) {
+        const int y = idx / n, x = idx - y * n;
+        double fv;
+        const double r = residual_at<Tu>(fp, up, (long)z * n * n + idx, x, y, z, n, n, nplanes, ix, iy, iz, fv);
+        rr += r * r;
+        ff += fv * fv;
+        lds[LX::at(y + 1, x + 1)] = (float)r;
+    }
+    block_sum2_to<NT>(rr, ff, partial + ((long)p * nplanes + z) * 2, red);
+    __syncthreads();
+    dst_pairs<float, LOGN, N / 2, LX, NT>(lds, L, sx, tid);
+    __syncthreads();
+    dst_pairs<float, LOGN, N / 2, LY, NT>(lds, L, sy, tid);
+    __syncthreads();
+    float* dst = out + p * bs + (long)z * n * n;
+    for (int idx = tid; idx < n * n; idx += NT) {
+        const int y = idx / n, x = idx - y * n;
+        dst[idx] = lds[LX::at(S::sigma(y + 1), S::sigma(x + 1))];
+    }
+}
+
+template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND>
+static hipError_t plane_k(const PlaneArgs& a) {
+    using C = PlaneCfg<LOGN>;
+    auto k = plane_dst_kernel<Tin, Tout, T, LOGN, ACCUM, KIND>;
+    const size_t lds = C::template lds_bytes<T>();
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    const BatchPtr<const Tin> in{(const Tin*)a.in, a.in_tab, a.bs_in};
+    const BatchPtr<Tout> out{(Tout*)a.out, a.out_tab, a.bs_out};
+    hipLaunchKernelGGL(k, dim3(a.nplanes, a.batch), dim3(C::NT), lds, a.stream, in, out, (T)a.sx, (T)a.sy,
+                       (const T*)a.tables, a.done);
+    return hipGetLastError();
+}
+
+template <typename Tin, typename Tout, typename T, int LOGN>
+static hipError_t plane_t(const PlaneArgs& a) {
+    if (a.kind == kDst)
+        return a.accum ? plane_k<Tin, Tout, T, LOGN, true, kDst>(a) : plane_k<Tin, Tout, T, LOGN, false, kDst>(a);
+    if constexpr (std::is_same_v<Tin, T> && std::is_same_v<Tout, T>) {
+        if (a.accum) return hipErrorInvalidValue;
+        if (a.kind == kRdftFwd) return plane_k<Tin, Tout, T, LOGN, false, kRdftFwd>(a);
+        if (a.kind == kRdftInv) return plane_k<Tin, Tout, T, LOGN, false, kRdftInv>(a);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename Tu, int LOGN>
+static hipError_t plane_resid_t(const PlaneResidArgs& a) {
+    using C = PlaneCfg<LOGN>;
+    auto k = plane_resid_dst_kernel<Tu, LOGN>;
+    const size_t lds = C::template lds_bytes<float>() + 16 + 2 * sizeof(double) * ((C::NT + 63) / 64);
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    const int per = (a.nplanes + kXcds - 1) / kXcds;
+    const BatchPtr<const double> f{a.f, a.f_tab, a.bs};
+    const BatchPtr<const Tu> u{(const Tu*)a.u, a.u_tab, a.bs};
+    hipLaunchKernelGGL(k, dim3(per * kXcds, a.batch), dim3(C::NT), lds, a.stream, f, u, a.out, a.partial, a.bs,
+                       a.nplanes, a.ix, a.iy, a.iz, (float)a.sx, (float)a.sy, (const float*)a.tables, a.done);
+    return hipGetLastError();
+}
+
+template <typename Tu>
+hipError_t launch_plane_resid(int logn, const PlaneResidArgs& a) {
+    switch (logn) {
+        case 1: return plane_resid_t<Tu, 1>(a);
+        case 2: return plane_resid_t<Tu, 2>(a);
+        case 3: return plane_resid_t<Tu, 3>(a);
+        case 4: return plane_resid_t<Tu, 4>(a);
+        case 5: return plane_resid_t<Tu, 5>(a);
+        case 6: return plane_resid_t<Tu, 6>(a);
+        case 7: return plane_resid_t<Tu, 7>(a);
+        default: return hipErrorInvalidValue;
+    }
+}
+template hipError_t launch_plane_resid<double>(int, const PlaneResidArgs&);
+
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_plane(int logn, const PlaneArgs& a) {
+    switch (logn) {
+        case 1: return plane_t<Tin, Tout, T, 1>(a);
+        case 2: return plane_t<Tin, Tout, T, 2>(a);
+        case 3: return plane_t<Tin, Tout, T, 3>(a);
+        case 4: return plane_t<Tin, Tout, T, 4>(a);
+        case 5: return plane_t<Tin, Tout, T, 5>(a);
+        case 6: return plane_t<Tin, Tout, T, 6>(a);
+        case 7: return plane_t<Tin, Tout, T, 7>(a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+bool plane_fits(int logn) {
+    if (logn < 1 || logn > 7) return false;
+    size_t N = (size_t)1 << logn, N1 = (size_t)1 << (logn / 2);
+    return sizeof(T) * (N * (N + N1)) <= 160 * 1024;
+}
+
+template hipError_t launch_plane<float, float, float>(int, const PlaneArgs&);
+template hipError_t launch_plane<double, double, double>(int, const PlaneArgs&);
+template hipError_t launch_plane<double, float, float>(int, const PlaneArgs&);
+template hipError_t launch_plane<float, double, float>(int, const PlaneArgs&);
+template bool plane_fits<float>(int);
+template bool plane_fits<double>(int);
+
+}  // namespace fdmgpu

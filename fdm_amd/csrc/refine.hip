@@ -44,15 +44,15 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* sh) {
 // A launch for already-converged problems costs a few microseconds.
 template <bool THREE, bool SCALED>
 __global__ __launch_bounds__(kResBlock) void residual_kernel(
-    const double* __restrict__ f, const double* __restrict__ u, float* __restrict__ r32,
+    const BatchPtr<const double> f, const BatchPtr<const double> u, float* __restrict__ r32,
     double* __restrict__ partial, const int nx, const int ny, const int nz, const double ix, const double iy,
     const double iz, const double* __restrict__ sy, const double* __restrict__ lx,
     const double* __restrict__ ux, const long bs, const int* __restrict__ done) {
     __shared__ double sh[2 * kResBlock];
     const int p = blockIdx.y;
     if (done && done[p]) return;
-    const double* fp = f + p * bs;
-    const double* up = u + p * bs;
+    const double* __restrict__ fp = f(p);
+    const double* __restrict__ up = u(p);
     float* rp = r32 ? r32 + p * bs : nullptr;
     const long pl = (long)nx * ny;
     const int nblk = gridDim.x;
@@ -127,14 +127,15 @@ constexpr long kResMaxBlocks = 1024;   // per problem
 hipError_t launch_residual(const ResidualArgs& a) {
     const unsigned nblk = (unsigned)residual_partials(a.nx, a.ny, a.nz);
     dim3 grid(nblk, a.batch);
+    const BatchPtr<const double> f{a.f, a.f_tab, a.bs}, u{a.u, a.u_tab, a.bs};
     if (a.nz > 1 || a.three) {
-        hipLaunchKernelGGL((residual_kernel<true, false>), grid, dim3(kResBlock), 0, a.stream, a.f, a.u, a.r32,
+        hipLaunchKernelGGL((residual_kernel<true, false>), grid, dim3(kResBlock), 0, a.stream, f, u, a.r32,
                            a.partial, a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, nullptr, nullptr, nullptr, a.bs, a.done);
     } else if (a.sy) {
-        hipLaunchKernelGGL((residual_kernel<false, true>), grid, dim3(kResBlock), 0, a.stream, a.f, a.u, a.r32,
+        hipLaunchKernelGGL((residual_kernel<false, true>), grid, dim3(kResBlock), 0, a.stream, f, u, a.r32,
                            a.partial, a.nx, a.ny, 1, a.ix, a.iy, 0.0, a.sy, a.lx, a.ux, a.bs, a.done);
     } else {
-        hipLaunchKernelGGL((residual_kernel<false, false>), grid, dim3(kResBlock), 0, a.stream, a.f, a.u, a.r32,
+        hipLaunchKernelGGL((residual_kernel<false, false>), grid, dim3(kResBlock), 0, a.stream, f, u, a.r32,
                            a.partial, a.nx, a.ny, 1, a.ix, a.iy, 0.0, nullptr, nullptr, nullptr, a.bs, a.done);
     }
     hipError_t e = hipGetLastError();

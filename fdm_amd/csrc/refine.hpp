@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "kernels.hpp"
+
 namespace fdmgpu {
 
 struct ResidualArgs {
@@ -24,6 +26,8 @@ struct ResidualArgs {
     double* resid;
     int it;
     hipStream_t stream;
+    const void* const* f_tab;   // per-problem pointers (device) or null
+    const void* const* u_tab;
 };
 
 hipError_t launch_residual(const ResidualArgs& a);

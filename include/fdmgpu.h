@@ -90,6 +90,26 @@ int fdmgpu_solve_device(fdmgpu_plan plan, void* d_ans, const void* d_rhs, void* 
  * (stride = field size) in one launch sequence.  No reference counterpart. */
 int fdmgpu_solve_batch_device(fdmgpu_plan plan, int count, void* d_ans, const void* d_rhs, void* stream);
 
+/* The same batch with problem p at d_ans_tab[p] / d_rhs_tab[p]: DEVICE arrays
+ * of `count` device pointers (problems in separate buffers).  For
+ * FDMGPU_F32_REFINE64 plans every ans pointer must differ from its rhs
+ * pointer.  No reference counterpart. */
+int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_tab, const void* const* d_rhs_tab,
+                              void* stream);
+
+/* Kernel variants (tile shapes) of a plan, as the autotuner chooses them:
+ * plane = 1 runs square planes that fit LDS through the plane kernel (0: x
+ * and y tile kernels); tile_x / tile_y = pencils per block of the tile
+ * kernels (16, 32, 64; 0 = default); zcr_gmul = lanes per z system
+ * multiplier (1, 2; 0 = 1).  Every variant solves the same system; results
+ * differ at rounding level.  No reference counterpart. */
+typedef struct fdmgpu_variant {
+    int plane, tile_x, tile_y, zcr_gmul;
+} fdmgpu_variant;
+int fdmgpu_plan_variants(fdmgpu_plan plan, int max, fdmgpu_variant* out, int* count);
+int fdmgpu_plan_set_variant(fdmgpu_plan plan, const fdmgpu_variant* variant);
+int fdmgpu_plan_get_variant(fdmgpu_plan plan, fdmgpu_variant* variant);
+
 /* Statistics of the plan's last solve (synchronises the stream it ran on).
  * iterations: FP32 inner solves (refine plans; 1 otherwise); rel_residual:
  * ||f - A u||/||f|| of the last problem (refine plans; -1 otherwise). */
@@ -124,6 +144,71 @@ int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resid
 #define FDMGPU_NUM_KERNELS 10
 int fdmgpu_set_timing(fdmgpu_plan plan, int enable);
 int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launches);
+
+/* ---- heterogeneous batches and serving ------------------------------------
+ * No reference counterpart: the reference solves one problem per call
+ * (LaplCube::solve).  A problem names its solver (the constructor arguments
+ * of LaplCube or LaplRect) and its device arrays. */
+#define FDMGPU_KIND_RECT 2
+#define FDMGPU_KIND_CUBE 3
+typedef struct fdmgpu_problem {
+    int kind;                 /* FDMGPU_KIND_CUBE or FDMGPU_KIND_RECT */
+    int precision;            /* FDMGPU_F32, FDMGPU_F64, FDMGPU_F32_REFINE64 */
+    int periodic;             /* mask as for fdmgpu_plan_cube / fdmgpu_plan_rect */
+    int nx, ny, nz;           /* nz = 1 for a rect */
+    double dx, dy, dz;        /* dz, lz unused for a rect */
+    double lx, ly, lz;
+    void* d_ans;              /* device array of the plan's element type; != d_rhs for refine */
+    const void* d_rhs;
+} fdmgpu_problem;
+
+/* Per-problem outcome. */
+typedef struct fdmgpu_result {
+    int state;                /* 0 queued, 1 on the GPU, 2 done */
+    int status;               /* FDMGPU_OK, or the error code of the solve */
+    int iterations;           /* FP32 inner solves (refine), 1 otherwise */
+    double rel_residual;      /* ||f - A u|| / ||f|| (refine), -1 otherwise */
+    double queue_ms;          /* submit -> launch */
+    double latency_ms;        /* submit -> completion observed */
+} fdmgpu_result;
+
+/* A fixed set of `count` problems of any shapes solved together: problems of
+ * one solver form one batch (one launch per stage), batches run concurrently
+ * on `nstreams` internal streams (0 = 4).  The first fdmgpu_group_solve runs
+ * eagerly; with FDMGPU_GROUP_GRAPH later calls replay a captured hipGraph of
+ * the whole sequence (one graph launch).  Kernel variants come from the
+ * autotuner cache when FDMGPU_TUNE_CACHE names one (see fdmgpu_tune). */
+typedef struct fdmgpu_group_t* fdmgpu_group;
+#define FDMGPU_GROUP_GRAPH 1
+int fdmgpu_group_create(fdmgpu_group* group, int count, const fdmgpu_problem* problems, int nstreams, int flags);
+int fdmgpu_group_solve(fdmgpu_group group, void* stream);
+/* After the solve completed (synchronises): result of problem i. */
+int fdmgpu_group_result(fdmgpu_group group, int i, fdmgpu_result* result);
+int fdmgpu_group_destroy(fdmgpu_group group);
+
+/* Serving: requests are admitted in arrival order while their scratch fits
+ * the HBM budget (an arena of `hbm_budget` bytes allocated once), batched per
+ * solver as they are admitted (continuous batching: requests arriving while
+ * others run form new batches on other streams) and completed
+ * asynchronously.  Solver tables stay resident between requests.  A request
+ * whose scratch alone exceeds the budget fails with FDMGPU_ERR_ADMISSION.
+ * Results of a request do not depend on what else is in flight. */
+typedef struct fdmgpu_server_t* fdmgpu_server;
+int fdmgpu_server_create(fdmgpu_server* server, unsigned long long hbm_budget, int max_batch, int nstreams);
+int fdmgpu_server_submit(fdmgpu_server server, const fdmgpu_problem* problem, unsigned long long* ticket);
+int fdmgpu_server_poll(fdmgpu_server server, unsigned long long ticket, fdmgpu_result* result);
+int fdmgpu_server_wait(fdmgpu_server server, unsigned long long ticket, fdmgpu_result* result);
+/* Blocks until every submitted request completed. */
+int fdmgpu_server_drain(fdmgpu_server server);
+/* Forgets completed requests' results (they are kept until then). */
+int fdmgpu_server_forget(fdmgpu_server server, unsigned long long ticket);
+int fdmgpu_server_destroy(fdmgpu_server server);
+
+/* Autotuner: times the kernel variants of every solver in `problems` (each
+ * solved `reps` times per variant on scratch copies) and records the fastest
+ * in the process-wide table, and in the JSON file `cache_path` when not NULL
+ * (read back by later processes through FDMGPU_TUNE_CACHE). */
+int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char* cache_path);
 
 /* Replaces ~LaplCube / ~LaplRect. */
 int fdmgpu_destroy(fdmgpu_plan plan);
