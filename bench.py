@@ -1,18 +1,26 @@
 #!/usr/bin/env python3
-"""Headline benchmark: LaplCube Dirichlet solves/s at 128^3 (interior 127^3).
+"""Headline benchmark: LaplCube Dirichlet solves/s at 128^3 (interior 127^3),
+FP32+refine to 1e-10 next to plain FP64, plus the serving latency the metric
+names and the other workloads as extra fields of the same JSON line.
 
-Metric (the north star's): solves/s of FP32+refine solves reaching
-||f - A u||_2/||f||_2 <= 1e-10 (FP64), with the plain FP64 GPU solve timed
-beside it, plus the CPU reference timed on the host cores.
+Legs (each can be skipped):
+  head  (the `value`): a step is one batched solve of `--batch` independent
+        127^3 problems (fdmgpu_solve_batch_device), inputs resident in HBM.
+        Inputs: seeded manufactured fields, u ~ U[-1,1) from a counter-based
+        splitmix64 (seed 3 + problem index), f = the 7-point discrete Laplacian
+        of u with zero Dirichlet boundary, geometry [-pi,pi]^3, h = 2pi/127,
+        l = 128 h.
+  c2    one 255^3 problem (seed 2): plain FP64 vs FP32+refine, error, iterations.
+  c3    512 problems, each axis n in {31, 63, 127} by seed 3, 25% 2-D LaplRect,
+        FP32+refine, kernel variants from the autotuner, all in one group
+        (one launch per stage and solver, replayed as one hipGraph).
+  c4    serving: Poisson arrivals (seeded exponential gaps) of the c3 mix plus
+        5% 255^3, FP32+refine, at 0.5x / 0.8x / 1.0x of the measured
+        saturation rate; p50/p99 latency from the scheduled arrival to the
+        completion; FP64 one-at-a-time solves of the same mix beside it.
+  cpu   the reference's LaplCube<double> on the host cores (bounded sample).
 
-A "step" is one batched solve of `--batch` independent 127^3 problems
-(fdmgpu_solve_batch_device), inputs already resident in HBM.  Inputs are the
-workload's seeded manufactured fields: u ~ U[-1,1) from a counter-based
-splitmix64 (seed s + problem index), f = the 7-point discrete Laplacian of u
-with zero Dirichlet boundary, geometry [-pi,pi]^3 with h = 2pi/127,
-l = 128 h.
-
-Prints ONE JSON line (see DESIGN.md for the byte model behind `roofline`).
+Prints ONE JSON line.
 """
 import argparse
 import json
@@ -53,6 +61,69 @@ def manufactured(n: int, seed: int):
     h = 2 * math.pi / n
     u = splitmix_field(seed, n ** 3).reshape(n, n, n)
     return u, lapl7(u, h), h, (n + 1) * h
+
+
+def lapl_aniso(u: np.ndarray, hs) -> np.ndarray:
+    """7-point (3-D) or 5-point (2-D) discrete Laplacian, zero Dirichlet boundary,
+    spacing hs[axis] per array axis."""
+    p = np.pad(u, 1)
+    out = np.zeros_like(u)
+    for ax in range(u.ndim):
+        lo = [slice(1, -1)] * u.ndim
+        hi = [slice(1, -1)] * u.ndim
+        lo[ax] = slice(0, -2)
+        hi[ax] = slice(2, None)
+        out += (p[tuple(lo)] + p[tuple(hi)] - 2.0 * u) / (hs[ax] * hs[ax])
+    return out
+
+
+def shape_geometry(kind: str, nx: int, ny: int, nz: int):
+    """[-pi,pi] per axis: h = 2pi/n, l = (n+1) h (the NSCube convention)."""
+    hx, hy, hz = 2 * math.pi / nx, 2 * math.pi / ny, 2 * math.pi / nz
+    if kind == "rect":
+        return (hx, hy, (nx + 1) * hx, (ny + 1) * hy)
+    return (hx, hy, hz, (nx + 1) * hx, (ny + 1) * hy, (nz + 1) * hz)
+
+
+def manufactured_shape(kind: str, nx: int, ny: int, nz: int, seed: int):
+    if kind == "rect":
+        u = splitmix_field(seed, nx * ny).reshape(ny, nx)
+        return u, lapl_aniso(u, (2 * math.pi / ny, 2 * math.pi / nx))
+    u = splitmix_field(seed, nx * ny * nz).reshape(nz, ny, nx)
+    return u, lapl_aniso(u, (2 * math.pi / nz, 2 * math.pi / ny, 2 * math.pi / nx))
+
+
+def uneven_mix(count: int, seed: int, big_frac: float = 0.0):
+    """The batched-uneven mix: per problem each axis n from {31, 63, 127}, a 2-D
+    LaplRect with probability 0.25; with probability big_frac a 255^3 cube."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        if big_frac and rng.random() < big_frac:
+            out.append(("cube", 255, 255, 255))
+        elif rng.random() < 0.25:
+            nx, ny = (int(v) for v in rng.choice([31, 63, 127], 2))
+            out.append(("rect", nx, ny, 1))
+        else:
+            nx, ny, nz = (int(v) for v in rng.choice([31, 63, 127], 3))
+            out.append(("cube", nx, ny, nz))
+    return out
+
+
+def make_problem(serve, shp, d_ans, d_rhs, precision):
+    kind, nx, ny, nz = shp
+    g = shape_geometry(kind, nx, ny, nz)
+    if kind == "rect":
+        return serve.rect_problem(*g, nx, ny, d_ans, d_rhs, precision=precision)
+    return serve.cube_problem(*g, nx, ny, nz, d_ans, d_rhs, precision=precision)
+
+
+def progress(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q)) if len(xs) else None
 
 
 def unixbench_score(ts):
@@ -145,6 +216,210 @@ def run_gpu(mode: str, n: int, batch: int, steps: int, warmup: int, f_host: np.n
     }
 
 
+def leg_c2(reps: int = 5):
+    """One 255^3 problem: plain FP64 vs FP32+refine (axis-tile kernels: a 256^2
+    plane does not fit LDS)."""
+    import fdm_amd
+    from fdm_amd import hip
+    n = 255
+    u, f = manufactured_shape("cube", n, n, n, 2)
+    g = shape_geometry("cube", n, n, n)
+    d_f = hip.DeviceArray.from_host(f)
+    out = {"workload": "LaplCube Dirichlet 255^3 (N=256), one problem, seed 2", "points": n ** 3}
+    st = hip.Stream()
+    res = {}
+    for mode in ("f64", "refine"):
+        s = fdm_amd.LaplCube(*g, n, n, n, precision=mode)
+        d_u = hip.DeviceArray(f.shape, np.float64)
+        s.solve(d_u, d_f, st)
+        st.synchronize()
+        s.set_timing(True)
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(st)
+        for _ in range(reps):
+            s.solve(d_u, d_f, st)
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        stats = s.stats()
+        ans = d_u.download()
+        it = stats.iterations if mode == "refine" else 1
+        res[mode] = ans
+        out[mode] = {"ms_per_solve": round(ms, 4), "solves_per_s": round(1e3 / ms, 2), "iterations": it,
+                     "err_vs_manufactured": float(np.linalg.norm(ans - u) / np.linalg.norm(u)),
+                     "model_GBs": round(48.0 * it * n ** 3 / (ms / 1e3) / 1e9, 1),
+                     "kernel_ms_per_solve": {k: round(v[0] / reps, 4) for k, v in s.timing().items()}}
+        if mode == "refine":
+            out[mode]["rel_residual"] = stats.rel_residual
+        del s
+    out["err_refine_vs_fp64_gpu"] = float(np.linalg.norm(res["refine"] - res["f64"]) / np.linalg.norm(res["f64"]))
+    out["refine_over_fp64"] = round(out["refine"]["solves_per_s"] / out["f64"]["solves_per_s"], 3)
+    return out
+
+
+def leg_c3(steps: int = 5, count: int = 512, tune_reps: int = 3):
+    """512 uneven problems, FP32+refine, autotuned variants, one group replayed
+    as a hipGraph."""
+    import fdm_amd
+    from fdm_amd import hip, serve
+    mix = uneven_mix(count, 3)
+    t0 = time.perf_counter()
+    bufs, probs, us = [], [], []
+    points = 0
+    for i, shp in enumerate(mix):
+        u, f = manufactured_shape(*shp, 1000 + i)
+        d_f = hip.DeviceArray.from_host(f)
+        d_u = hip.DeviceArray(f.shape, np.float64)
+        bufs.append((d_u, d_f))
+        probs.append(make_problem(serve, shp, d_u, d_f, fdm_amd.F32_REFINE64))
+        if i < 8:
+            us.append(u)
+        points += f.size
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    serve.tune(probs, reps=tune_reps)
+    tune_s = time.perf_counter() - t0
+    grp = serve.Group(probs, nstreams=4, graph=True)
+    st = hip.Stream()
+    grp.solve(st)          # eager
+    grp.solve(st)          # captured, first replay
+    st.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(st)
+    for _ in range(steps):
+        grp.solve(st)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    res = [grp.result(i) for i in range(count)]
+    errs = [float(np.linalg.norm(bufs[i][0].download() - us[i]) / np.linalg.norm(us[i])) for i in range(len(us))]
+    iters = [r.iterations for r in res]
+    out = {
+        "workload": f"{count} problems, each axis n in {{31,63,127}} by seed 3, 25% 2-D LaplRect, FP32+refine "
+                    f"to 1e-10; one group (one launch per stage and solver, 4 streams) replayed as a hipGraph",
+        "problems": count, "solvers": len({m for m in mix}), "points": points,
+        "ms_per_step": round(ms, 3), "solves_per_s": round(count / (ms / 1e3), 1),
+        "points_per_s": round(points / (ms / 1e3), 1),
+        "model_GBs": round(sum(48.0 * it * float(np.prod([d for d in m[1:]])) for it, m in zip(iters, mix))
+                           / (ms / 1e3) / 1e9, 1),
+        "iterations_max": max(iters), "max_rel_residual": max(r.rel_residual for r in res),
+        "all_converged": all(r.status == 0 for r in res), "err_vs_manufactured_first8": max(errs),
+        "tuned": True, "tune_s": round(tune_s, 2), "gen_s": round(gen_s, 2),
+    }
+    grp.close()
+    return out
+
+
+def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 32, seed: int = 4,
+           fp64_sample: int = 120):
+    """Serving: Poisson arrivals of the uneven mix + 5% 255^3 (FP32+refine)."""
+    import fdm_amd
+    from fdm_amd import hip, serve
+    rng = np.random.default_rng(seed)
+    depth = 24                   # in-flight requests per shape (own rhs and ans buffers each)
+    # request streams first, so every buffer exists before anything is timed
+    warm = uneven_mix(600, seed, big_frac=0.05)
+    fracs = (0.5, 0.8, 1.0)
+    fp64_reqs = uneven_mix(fp64_sample, seed + 7, big_frac=0.05)
+    shapes = {}
+
+    def prepare(shps):
+        for shp in shps:
+            if shp in shapes:
+                continue
+            u, f = manufactured_shape(*shp, 2000 + len(shapes))
+            tmpl = hip.DeviceArray.from_host(f)
+            slots = []
+            for _ in range(depth):
+                d_f = hip.DeviceArray(f.shape, np.float64)
+                hip.check(hip.rt().hipMemcpy(d_f.ptr, tmpl.ptr, d_f.nbytes, hip.D2D), "copy")
+                slots.append((hip.DeviceArray(f.shape, np.float64), d_f))
+            shapes[shp] = {"f": f, "tmpl": tmpl, "slots": slots, "busy": [None] * depth, "next": 0}
+
+    prepare(warm + fp64_reqs)
+    S = serve.Server(hbm_budget=int(budget_gb * (1 << 30)), max_batch=max_batch, nstreams=4)
+
+    def submit(shp, t_sched):
+        e = shapes[shp]
+        j = e["next"]
+        e["next"] = (j + 1) % depth
+        if e["busy"][j] is not None:      # slot reuse: its previous request must be done
+            S.wait(e["busy"][j])
+        d_u, d_f = e["slots"][j]
+        t_sub = time.perf_counter()
+        tk = S.submit(make_problem(serve, shp, d_u, d_f, fdm_amd.F32_REFINE64))
+        e["busy"][j] = tk
+        return tk, t_sub - t_sched
+
+    def run(reqs, gaps):
+        """Submit at the scheduled times; latency = client delay + server latency."""
+        prepare(reqs)
+        t_start = time.perf_counter()
+        sched = t_start + np.concatenate([[0.0], np.cumsum(gaps)[:-1]]) if gaps is not None else None
+        tick = []
+        for i, shp in enumerate(reqs):
+            ts = sched[i] if sched is not None else time.perf_counter()
+            while time.perf_counter() < ts:
+                pass
+            tick.append(submit(shp, ts))
+        t_subs = time.perf_counter()
+        S.drain()
+        t_end = time.perf_counter()
+        run.submit_s = t_subs - t_start
+        lat, resid, ok = [], [], True
+        for tk, delay in tick:
+            r = S.poll(tk)
+            lat.append(r.latency_ms + 1e3 * delay)
+            resid.append(r.rel_residual)
+            ok = ok and r.status == 0
+            S.forget(tk)
+        for e in shapes.values():
+            e["busy"] = [None] * depth
+        return t_end - t_start, lat, resid, ok
+
+    # warm-up (plans, tables, pinned batches, iteration counts), then saturation:
+    # a burst of requests all submitted at t = 0
+    run(warm[:200], None)
+    dt, lat, _, _ = run(warm[200:600], None)
+    sat = 400 / dt
+    burst_submit_per_s = 400 / run.submit_s
+    out = {"workload": "requests of the uneven mix (each axis n in {31,63,127}, 25% 2-D) + 5% 255^3, "
+                       "FP32+refine to 1e-10, Poisson arrivals (seeded exponential gaps)",
+           "hbm_budget_gb": budget_gb, "max_batch": max_batch, "streams": 4,
+           "saturation_solves_per_s": round(sat, 1), "burst_client_submit_per_s": round(burst_submit_per_s, 1),
+           "rates": []}
+    for frac in fracs:
+        rate = frac * sat
+        n = max(50, int(rate * duration_s))
+        reqs = uneven_mix(n, seed + int(frac * 100), big_frac=0.05)
+        gaps = rng.exponential(1.0 / rate, n)
+        dt, lat, resid, ok = run(reqs, gaps)
+        out["rates"].append({"offered_per_s": round(rate, 1), "requests": n,
+                             "achieved_solves_per_s": round(n / dt, 1),
+                             "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
+                             "max_rel_residual": max(resid), "all_converged": ok})
+    S.close()
+    # FP64 one request at a time (no batching, no overlap), same mix
+    plans, times = {}, []
+    st = hip.Stream()
+    for shp in fp64_reqs:
+        d_u, d_f = shapes[shp]["slots"][0]
+        if shp not in plans:
+            kind, nx, ny, nz = shp
+            g = shape_geometry(*shp)
+            plans[shp] = (fdm_amd.LaplRect(*g, nx, ny, precision="f64") if kind == "rect"
+                          else fdm_amd.LaplCube(*g, nx, ny, nz, precision="f64"))
+            plans[shp].solve(d_u, d_f, st)   # warm-up
+            st.synchronize()
+        t = time.perf_counter()
+        plans[shp].solve(d_u, d_f, st)
+        st.synchronize()
+        times.append(1e3 * (time.perf_counter() - t))
+    out["fp64_one_at_a_time"] = {"requests": len(times), "solves_per_s": round(1e3 * len(times) / sum(times), 1),
+                                 "p50_ms": round(pct(times, 50), 3), "p99_ms": round(pct(times, 99), 3)}
+    return out
+
+
 def cpu_baseline(n: int, f0: np.ndarray, budget_s: float = 15.0):
     """The reference's own LaplCube<double,false> (oracle/_ref, built from the reference
     sources) when present, else the oracle restatement; OpenMP over the host cores."""
@@ -199,6 +474,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mode", choices=("both", "refine", "f64"), default="both",
                     help="which GPU legs to run (profiling runs use one)")
+    ap.add_argument("--legs", default="head,c2,c3,c4,cpu",
+                    help="comma list of legs: head (the value; always run), c2, c3, c4, cpu")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving rate")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (calibrates the HBM "
                          "FETCH_SIZE/WRITE_SIZE counters against a copy of known size) and exit")
@@ -214,9 +492,12 @@ def main():
         print(json.dumps({"calib_copy_bytes": nb, "copies": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only: the batched solve shards by whole "
-                                   "problems across GPUs (see DESIGN.md); run with --gpus 1"}))
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding "
+                                   "by whole problems is a design note, not implemented)"}))
         sys.exit(2)
+    legs = set(args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
     n, B = args.n, args.batch
 
     # inputs: B seeded manufactured problems (seed 3 + i), as in the batched workload
@@ -235,8 +516,10 @@ def main():
                           "iterations": r["iterations"],
                           "kernel_ms_per_step": {k: v[0] / args.steps for k, v in r["timing"].items()}}))
         return
+    progress(f"head: {B} x {n}^3 FP32+refine and FP64")
     res_r = run_gpu("refine", n, B, args.steps, args.warmup, F)
     res_d = run_gpu("f64", n, B, args.steps, args.warmup, F)
+    fcpu = F[0].copy()
 
     solves = B * args.steps
     value = solves / (res_r["wall_ms"] / 1e3)
@@ -254,7 +537,7 @@ def main():
     err_r = float(np.linalg.norm(res_r["u0"] - res_d["u0"]) / np.linalg.norm(res_d["u0"]))
 
     out = {
-        "metric": "solves/s @128^3 to 1e-10 rel (FP32+refine)",
+        "metric": "solves/s @128\u00b3 to 1e-10 rel (FP32+refine vs FP64); p50/p99 serve latency",
         "value": round(value, 2),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -285,8 +568,21 @@ def main():
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tm.items()},
         "fp64_kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in res_d["timing"].items()},
     }
-    if not args.no_cpu:
-        cb, ans_cpu = cpu_baseline(n, F[0], args.cpu_budget)
+    del F
+    if "c2" in legs:
+        progress("c2: 255^3 FP64 vs FP32+refine")
+        out["single_255"] = leg_c2()
+    if "c3" in legs:
+        progress("c3: 512 uneven problems (tune, group, graph)")
+        out["batched_uneven"] = leg_c3()
+    if "c4" in legs:
+        progress("c4: serving at 3 arrival rates")
+        out["serve"] = leg_c4(duration_s=args.serve_seconds)
+        out["p50_ms"] = [r["p50_ms"] for r in out["serve"]["rates"]]
+        out["p99_ms"] = [r["p99_ms"] for r in out["serve"]["rates"]]
+    if "cpu" in legs:
+        progress("cpu: reference LaplCube<double> on the host cores")
+        cb, ans_cpu = cpu_baseline(n, fcpu, args.cpu_budget)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["err_fp64_gpu_vs_cpu"] = float(np.linalg.norm(res_d["u0"] - ans_cpu) / np.linalg.norm(ans_cpu))
         out["err_refine_vs_cpu"] = float(np.linalg.norm(res_r["u0"] - ans_cpu) / np.linalg.norm(ans_cpu))

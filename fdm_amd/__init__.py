@@ -35,7 +35,7 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_rect_set_scales", "fd
            "fdmgpu_plan_variants", "fdmgpu_plan_set_variant", "fdmgpu_plan_get_variant", "fdmgpu_get_stats",
            "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_group_create",
            "fdmgpu_group_solve", "fdmgpu_group_result", "fdmgpu_group_destroy", "fdmgpu_server_create",
-           "fdmgpu_server_submit", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
+           "fdmgpu_server_submit", "fdmgpu_server_set_refine", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
            "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_destroy", "fdmgpu_strerror",
            "fdmgpu_version")
 
@@ -121,6 +121,7 @@ def _lib_handle():
         L.fdmgpu_group_destroy.argtypes = [vp]
         L.fdmgpu_server_create.argtypes = [C.POINTER(vp), ull, i, i]
         L.fdmgpu_server_submit.argtypes = [vp, C.POINTER(Problem), C.POINTER(ull)]
+        L.fdmgpu_server_set_refine.argtypes = [vp, d, i]
         L.fdmgpu_server_poll.argtypes = [vp, ull, C.POINTER(Result)]
         L.fdmgpu_server_wait.argtypes = [vp, ull, C.POINTER(Result)]
         L.fdmgpu_server_drain.argtypes = [vp]

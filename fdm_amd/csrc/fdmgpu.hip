@@ -402,8 +402,10 @@ __global__ void add_f32_to_f64(const BatchPtr<double> u, const float* __restrict
 //   finally:      ||f - A u||/||f|| of the last iterate.
 // A problem whose residual reached rtol is flagged on the device; every later
 // launch skips it, so the host never synchronises inside a solve.
+// first > 1 continues a refinement whose iterate (in ans) after `first - 1`
+// inner solves did not reach the tolerance; last = the final inner solve.
 int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const double* rhs, Tab rhs_tab,
-                 int count, hipStream_t s) {
+                 int count, hipStream_t s, int first, int last) {
     int rc = work_reserve(P, W, count);
     if (rc) return rc;
     const long elems = (long)P->nx * P->ny * P->nz;
@@ -422,8 +424,7 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
     }
     {
         Timed tm(P, FDMGPU_K_OTHER, s);
-        HIPCHK(launch_fill_int(W.done, count, 0, s));
-        HIPCHK(launch_fill_int(W.iters, count, 0, s));
+        HIPCHK(launch_fill_int(W.done, W.iters, count, 0, s));
     }
     ResidualArgs ra{};
     ra.f = rhs;
@@ -455,7 +456,13 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
     // computes r = f - A u itself; convergence of the iterate after solve i is
     // certified by a norm-only FP64 residual pass.  Other shapes: the residual
     // pass writes r32 (FP32) for the next solve and certifies at the same time.
-    for (int it = 1; it <= P->max_iter; ++it) {
+    if (first > 1 && !fused) {   // the next solve's rhs: r32 = f - A u of the current iterate
+        ra.it = first - 1;
+        ra.r32 = W.r32;
+        Timed tm(P, FDMGPU_K_RESIDUAL, s);
+        HIPCHK(launch_residual(ra));
+    }
+    for (int it = first; it <= last; ++it) {
         if (it == 1) {
             rc = stage_fwd<double, float>(P, rhs, rhs_tab, W.r32, nullptr, count, s, W.done);
         } else if (fused) {
@@ -466,7 +473,7 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
                 Timed tm(P, FDMGPU_K_XY_FWD_RESID, s);
                 HIPCHK(launch_plane_resid<double>(P->logx, pa));
             }
-            if (it == 2) {   // the iterate after solve 1 was not certified
+            if (it == 2 && first == 1) {   // the iterate after solve 1 was not certified
                 Timed tm(P, FDMGPU_K_RESIDUAL, s);
                 HIPCHK(launch_check(W.partial, P->nz, P->rtol, W.done, W.iters, W.resid, 1, count, s));
             }
@@ -487,9 +494,9 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
             rc = 0;
         }
         if (rc) return rc;
-        if (fused && it == 1 && P->max_iter > 1) continue;   // solve 2's fused pass checks it
+        if (fused && it == 1 && last > 1) continue;   // solve 2's fused pass checks it
         ra.it = it;
-        ra.r32 = (fused || it == P->max_iter) ? nullptr : W.r32;
+        ra.r32 = (fused || it == last) ? nullptr : W.r32;
         Timed tm(P, FDMGPU_K_RESIDUAL, s);
         HIPCHK(launch_residual(ra));
     }
@@ -507,6 +514,11 @@ int solve_device_impl(fdmgpu_plan_t* P, void* ans, Tab ans_tab, const void* rhs,
 
 int fdmgpu::plan_solve(fdmgpu_plan_t* P, Work& W, void* ans, Tab ans_tab, const void* rhs, Tab rhs_tab, int count,
                        hipStream_t s) {
+    return plan_solve_iters(P, W, ans, ans_tab, rhs, rhs_tab, count, s, 1, P->max_iter);
+}
+
+int fdmgpu::plan_solve_iters(fdmgpu_plan_t* P, Work& W, void* ans, Tab ans_tab, const void* rhs, Tab rhs_tab,
+                             int count, hipStream_t s, int first, int last) {
     HIPCHK(hipSetDevice(P->device));
     if (count <= 0) return FDMGPU_OK;
     // the launchers check hipGetLastError(); drop a stale error of an earlier,
@@ -515,7 +527,9 @@ int fdmgpu::plan_solve(fdmgpu_plan_t* P, Work& W, void* ans, Tab ans_tab, const 
     switch (P->precision) {
         case FDMGPU_F32: return solve_plain<float>(P, ans, ans_tab, rhs, rhs_tab, count, s);
         case FDMGPU_F64: return solve_plain<double>(P, ans, ans_tab, rhs, rhs_tab, count, s);
-        default: return solve_refine(P, W, (double*)ans, ans_tab, (const double*)rhs, rhs_tab, count, s);
+        default:
+            if (first < 1 || last < first || last > 64) return FDMGPU_ERR_INVALID_ARG;
+            return solve_refine(P, W, (double*)ans, ans_tab, (const double*)rhs, rhs_tab, count, s, first, last);
     }
 }
 

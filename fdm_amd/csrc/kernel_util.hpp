@@ -4,16 +4,26 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <mutex>
+#include <unordered_map>
 
 namespace fdmgpu {
 
 // A kernel that uses more than 64 KB of dynamic LDS (gfx950: 160 KB per CU)
-// has to opt in.
+// has to opt in; the opt-in is made once per kernel and size (it is a host
+// call on the launch path of every solve otherwise).
 template <typename K>
 inline hipError_t allow_lds(K kernel, size_t bytes) {
     if (bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes);
+    static std::mutex mu;
+    static std::unordered_map<const void*, size_t> done;
+    const void* k = reinterpret_cast<const void*>(kernel);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(k);
+    if (it != done.end() && it->second >= bytes) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) done[k] = bytes;
+    return e;
 }
 
 constexpr size_t kLdsBytes = 160 * 1024;

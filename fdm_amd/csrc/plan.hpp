@@ -99,6 +99,10 @@ void work_free(Work& W);
 // ans + p*elems / rhs + p*elems, or at ans_tab[p] / rhs_tab[p] (device tables).
 int plan_solve(fdmgpu_plan_t* P, Work& W, void* ans, Tab ans_tab, const void* rhs, Tab rhs_tab, int count,
                hipStream_t s);
+// FP32+refine plans: inner solves first..last only (first > 1 continues from
+// the iterate in ans, whose residual after first-1 solves missed the tolerance).
+int plan_solve_iters(fdmgpu_plan_t* P, Work& W, void* ans, Tab ans_tab, const void* rhs, Tab rhs_tab, int count,
+                     hipStream_t s, int first, int last);
 // Elements of one problem.
 inline long plan_elems(const fdmgpu_plan_t* P) { return (long)P->nx * P->ny * P->nz; }
 // Error text of the last failure on this thread (FDMGPU_ERR_DEVICE details).

@@ -117,9 +117,12 @@ __global__ __launch_bounds__(kResBlock) void check_kernel(const double* __restri
     }
 }
 
-__global__ void fill_int_kernel(int* __restrict__ a, const int n, const int v) {
+__global__ void fill_int_kernel(int* __restrict__ a, int* __restrict__ b, const int n, const int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
+    if (i < n) {
+        a[i] = v;
+        if (b) b[i] = v;
+    }
 }
 
 constexpr long kResMaxBlocks = 1024;   // per problem
@@ -161,9 +164,9 @@ long residual_partials(int nx, int ny, int nz) {
     return b < kResMaxBlocks ? b : kResMaxBlocks;
 }
 
-hipError_t launch_fill_int(int* a, int n, int v, hipStream_t s) {
+hipError_t launch_fill_int(int* a, int* b, int n, int v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, n, v);
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n, v);
     return hipGetLastError();
 }
 

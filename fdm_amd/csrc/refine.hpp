@@ -35,6 +35,7 @@ hipError_t launch_residual(const ResidualArgs& a);
 hipError_t launch_check(const double* partial, int nblk, double tol, int* done, int* iters, double* resid, int it,
                         int batch, hipStream_t s);
 long residual_partials(int nx, int ny, int nz);
-hipError_t launch_fill_int(int* a, int n, int v, hipStream_t s);
+// a[i] = b[i] = v for i < n (b may be null); a kernel, so it can be captured in a graph
+hipError_t launch_fill_int(int* a, int* b, int n, int v, hipStream_t s);
 
 }  // namespace fdmgpu

@@ -13,7 +13,11 @@
 //    their scratch fits an arena of the HBM budget, launches a batch per
 //    solver as requests arrive (continuous batching across streams) and
 //    completes them when the batch's event fires.  Solver plans (twiddle and
-//    eigenvalue tables) stay resident.
+//    eigenvalue tables) stay resident.  An FP32+refine batch launches as many
+//    inner solves as that solver's requests have needed so far; a request
+//    still above the tolerance continues from its iterate in a follow-up
+//    batch (up to the plan's max_iter), so no launches are spent on
+//    converged problems.
 //  * fdmgpu_tune: times every kernel variant of a shape and keeps the fastest
 //    (process-wide table, optionally a JSON file read through
 //    FDMGPU_TUNE_CACHE).
@@ -400,6 +404,7 @@ struct Req {
     unsigned long long ticket = 0;
     fdmgpu_problem p{};
     Key key{};
+    int its_done = 0;          // inner solves already applied (continuation when > 0)
     Clock::time_point t_submit, t_launch, t_done;
     fdmgpu_result res{};
 };
@@ -408,6 +413,7 @@ struct Batch {
     std::vector<Req*> reqs;
     fdmgpu_plan plan = nullptr;
     Key key{};
+    int first = 1, last = 1;   // inner solves of this launch (refine)
     Work W;
     char* mem = nullptr;
     size_t bytes = 0;
@@ -433,6 +439,9 @@ struct fdmgpu_server_t {
     std::deque<Req*> queue;
     std::unordered_map<unsigned long long, Req*> reqs;
     std::map<Key, fdmgpu_plan> plans;
+    std::map<Key, int> typical;   // inner solves the solver's requests needed so far
+    double rtol = 1e-10;          // refinement settings of the server's plans
+    int max_iter = 4;
     std::vector<Batch*> running, spare;
     std::thread worker;
     bool stop = false;
@@ -482,7 +491,8 @@ int launch_batch(fdmgpu_server_t* S, Batch* b) {
     }
     hipStream_t s = S->streams[b->stream];
     SRV_HIPCHK(hipMemcpyAsync(d_tab, b->h_tab, sizeof(void*) * 2 * n, hipMemcpyHostToDevice, s));
-    int rc = plan_solve(b->plan, b->W, nullptr, (Tab)d_tab, nullptr, (Tab)(d_tab + n), n, s);
+    int rc = plan_solve_iters(b->plan, b->W, nullptr, (Tab)d_tab, nullptr, (Tab)(d_tab + n), n, s, b->first,
+                              b->last);
     if (rc) return rc;
     if (b->plan->precision == FDMGPU_F32_REFINE64) {
         SRV_HIPCHK(hipMemcpyAsync(b->h_iters, b->W.iters, sizeof(int) * n, hipMemcpyDeviceToHost, s));
@@ -505,10 +515,11 @@ void worker_main(fdmgpu_server_t* S) {
         while (!S->queue.empty()) {
             Req* head = S->queue.front();
             const Key k = head->key;
+            const int its = head->its_done;
             std::vector<Req*> take;
             for (Req* r : S->queue) {
                 if ((int)take.size() >= S->max_batch) break;
-                if (!(r->key < k) && !(k < r->key)) take.push_back(r);
+                if (!(r->key < k) && !(k < r->key) && r->its_done == its) take.push_back(r);
             }
             char* mem = nullptr;
             size_t bytes = 0;
@@ -528,6 +539,12 @@ void worker_main(fdmgpu_server_t* S) {
             b->reqs = take;
             b->key = k;
             b->plan = S->plans[k];
+            {
+                const int maxit = b->plan->max_iter;
+                const int typ = S->typical.count(k) ? S->typical[k] : maxit;
+                b->first = its + 1;
+                b->last = std::min(maxit, std::max(b->first, typ));
+            }
             b->mem = mem;
             b->bytes = bytes;
             int best = 0;
@@ -537,7 +554,7 @@ void worker_main(fdmgpu_server_t* S) {
             S->stream_load[best]++;
             for (Req* r : take) {
                 S->queue.erase(std::find(S->queue.begin(), S->queue.end(), r));
-                r->t_launch = Clock::now();
+                if (r->its_done == 0) r->t_launch = Clock::now();
                 r->res.state = 1;
             }
             lk.unlock();
@@ -571,19 +588,31 @@ void worker_main(fdmgpu_server_t* S) {
             }
             const Clock::time_point now = Clock::now();
             const bool refine = b->plan->precision == FDMGPU_F32_REFINE64;
+            std::vector<Req*> cont;
             for (size_t j = 0; j < b->reqs.size(); j++) {
                 Req* r = b->reqs[j];
+                const bool ok = q == hipSuccess;
+                const bool converged = !refine || b->h_resid[j] <= b->plan->rtol;
+                if (ok && refine) {
+                    int& typ = S->typical[b->key];
+                    typ = std::max(typ, converged ? b->h_iters[j] : b->last + 1);
+                }
+                if (ok && !converged && b->last < b->plan->max_iter) {   // continue from the iterate
+                    r->its_done = b->last;
+                    cont.push_back(r);
+                    continue;
+                }
                 r->t_done = now;
                 r->res.state = 2;
-                r->res.status = q == hipSuccess ? FDMGPU_OK : FDMGPU_ERR_DEVICE;
+                r->res.status = ok ? FDMGPU_OK : FDMGPU_ERR_DEVICE;
                 r->res.iterations = refine ? b->h_iters[j] : 1;
                 r->res.rel_residual = refine ? b->h_resid[j] : -1.0;
-                if (refine && r->res.status == FDMGPU_OK && !(r->res.rel_residual <= b->plan->rtol))
-                    r->res.status = FDMGPU_ERR_NOT_CONVERGED;
-                r->res.queue_ms = ms_between(r->t_submit, r->t_launch);
+                if (refine && r->res.status == FDMGPU_OK && !converged) r->res.status = FDMGPU_ERR_NOT_CONVERGED;
+                if (r->its_done == 0) r->res.queue_ms = ms_between(r->t_submit, r->t_launch);
                 r->res.latency_ms = ms_between(r->t_submit, r->t_done);
                 S->outstanding--;
             }
+            for (auto it = cont.rbegin(); it != cont.rend(); ++it) S->queue.push_front(*it);   // ahead of new work
             S->arena.release(b->mem, b->bytes);
             S->stream_load[b->stream]--;
             b->reqs.clear();
@@ -644,6 +673,7 @@ int fdmgpu_server_submit(fdmgpu_server S, const fdmgpu_problem* p, unsigned long
         (void)hipSetDevice(S->device);
         rc = make_plan(k, &plan);
         if (rc) return rc;
+        if (k.precision == FDMGPU_F32_REFINE64) (void)fdmgpu_set_refine(plan, S->rtol, S->max_iter);
         S->plans[k] = plan;
     }
     Req* r = new Req();
@@ -659,6 +689,16 @@ int fdmgpu_server_submit(fdmgpu_server S, const fdmgpu_problem* p, unsigned long
     *ticket = r->ticket;
     lk.unlock();
     S->cv_work.notify_one();
+    return FDMGPU_OK;
+}
+
+int fdmgpu_server_set_refine(fdmgpu_server S, double rel_tol, int max_iter) {
+    if (!S || !(rel_tol > 0) || max_iter < 1 || max_iter > 64) return FDMGPU_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->rtol = rel_tol;
+    S->max_iter = max_iter;
+    for (auto& kv : S->plans)
+        if (kv.first.precision == FDMGPU_F32_REFINE64) (void)fdmgpu_set_refine(kv.second, rel_tol, max_iter);
     return FDMGPU_OK;
 }
 

@@ -75,6 +75,9 @@ class Server:
         _check(_lib_handle().fdmgpu_server_submit(self._h, C.byref(problem), C.byref(t)), "server_submit")
         return t.value
 
+    def set_refine(self, rel_tol: float = 1e-10, max_iter: int = 4):
+        _check(_lib_handle().fdmgpu_server_set_refine(self._h, rel_tol, max_iter), "server_set_refine")
+
     def poll(self, ticket: int) -> Result:
         r = Result()
         _check(_lib_handle().fdmgpu_server_poll(self._h, ticket, C.byref(r)), "server_poll")

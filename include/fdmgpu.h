@@ -196,6 +196,9 @@ int fdmgpu_group_destroy(fdmgpu_group group);
 typedef struct fdmgpu_server_t* fdmgpu_server;
 int fdmgpu_server_create(fdmgpu_server* server, unsigned long long hbm_budget, int max_batch, int nstreams);
 int fdmgpu_server_submit(fdmgpu_server server, const fdmgpu_problem* problem, unsigned long long* ticket);
+/* Refinement tolerance and inner-solve limit of the server's FP32+refine
+ * solvers (defaults 1e-10, 4); applies to requests launched afterwards. */
+int fdmgpu_server_set_refine(fdmgpu_server server, double rel_tol, int max_iter);
 int fdmgpu_server_poll(fdmgpu_server server, unsigned long long ticket, fdmgpu_result* result);
 int fdmgpu_server_wait(fdmgpu_server server, unsigned long long ticket, fdmgpu_result* result);
 /* Blocks until every submitted request completed. */

@@ -257,3 +257,35 @@ def test_tune_writes_cache_and_applies(fdm, hip, srv, tmp_path):
     line = [l for l in text.splitlines() if l.strip().startswith(key)][0]
     v = s.variant
     assert f'"plane": {v[0]}, "tile_x": {v[1]}, "tile_y": {v[2]}, "zcr_gmul": {v[3]}' in line
+
+
+@pytest.mark.parametrize("shape", [(31, 31, 31), (63, 31, 15)])
+def test_server_continuation_matches_uninterrupted(fdm, hip, srv, shape):
+    """A request that needs more inner solves than its solver's earlier requests
+    continues from its iterate in a follow-up batch; the answer is bitwise the
+    uninterrupted solve's (fused plane path and the tile path)."""
+    nx, ny, nz = shape
+    say("server continuation", shape)
+    g = geom_cube(nx, ny, nz)
+    u, f = manufactured((nz, ny, nx), 41)
+    s = fdm.LaplCube(*g, nx, ny, nz, precision="refine")
+    s.set_refine(1e-30, 2)
+    ans = np.zeros_like(f)
+    with pytest.raises(fdm.FdmGpuError):        # 2 inner solves cannot reach 1e-30
+        s.solve(ans, f)
+    r2 = s.stats().rel_residual
+    tol = r2 / 2                                # 2 solves miss it, 3 reach it
+    s.set_refine(tol, 4)
+    ref = np.zeros_like(f)
+    s.solve(ref, f)
+    assert s.stats().iterations == 3
+    d_f = hip.DeviceArray.from_host(f)
+    d_a, d_b = hip.DeviceArray(f.shape, np.float64), hip.DeviceArray(f.shape, np.float64)
+    S = srv.Server(hbm_budget=256 << 20, max_batch=4)
+    r = S.wait(S.submit(srv.cube_problem(*g, nx, ny, nz, d_a, d_f)))   # the solver needs 2 at 1e-10
+    assert r.status == 0 and r.iterations == 2
+    S.set_refine(tol, 4)
+    r = S.wait(S.submit(srv.cube_problem(*g, nx, ny, nz, d_b, d_f)))   # launched with 2, continued to 3
+    assert r.status == 0 and r.iterations == 3 and r.rel_residual <= tol
+    assert np.array_equal(d_b.download(), ref)
+    S.close()
