@@ -59,10 +59,11 @@ class Stats(C.Structure):
 
 class Variant(C.Structure):
     """fdmgpu_variant: kernel variant (tile shapes) of a plan."""
-    _fields_ = [("plane", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("zcr_gmul", C.c_int)]
+    _fields_ = [("plane", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("zcr_gmul", C.c_int),
+                ("fuse", C.c_int)]
 
     def astuple(self):
-        return (self.plane, self.tile_x, self.tile_y, self.zcr_gmul)
+        return (self.plane, self.tile_x, self.tile_y, self.zcr_gmul, self.fuse)
 
 
 class Problem(C.Structure):

@@ -47,8 +47,8 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void xaxis_dst_kerne
     T* lds = reinterpret_cast<T*>(smem_raw);
     const int tid = threadIdx.x;
     const long r0 = (long)blockIdx.x * TILE;
-    const Tin* __restrict__ src = in(blockIdx.y) + r0 * n;
-    Tout* __restrict__ dst = out(blockIdx.y) + r0 * n;
+    const auto src = as_global(in(blockIdx.y) + r0 * n);
+    const auto dst = as_global(out(blockIdx.y) + r0 * n);
     const int rows_here = (int)((nrows - r0) < TILE ? (nrows - r0) : TILE);
 
     DstLane<T, LOGN> L;
@@ -86,8 +86,8 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void yaxis_dst_kerne
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TILE;
     const long poff = (long)blockIdx.y * n * nx;
-    const Tin* __restrict__ src = in(blockIdx.z) + poff;
-    Tout* __restrict__ dst = out(blockIdx.z) + poff;
+    const auto src = as_global(in(blockIdx.z) + poff);
+    const auto dst = as_global(out(blockIdx.z) + poff);
 
     DstLane<T, LOGN> L;
     L.init(tables, tid & (S::N1 - 1));

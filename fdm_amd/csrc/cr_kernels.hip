@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void zcr_kernel(const BatchPtr<Ts> data, const
     const long m = wave * W + s;
     const bool valid = m < plane;
     const long mm = valid ? m : 0;
-    Ts* __restrict__ base = data(blockIdx.y) + mm;
+    const auto base = as_global(data(blockIdx.y) + mm);
     const int ky = (int)(mm / nx), kx = (int)(mm - (long)ky * nx);
 
     T v[C];
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void zcrp_kernel(const BatchPtr<Ts> data, cons
     const long m = wave * W + s;
     const bool valid = m < plane;
     const long mm = valid ? m : 0;
-    Ts* __restrict__ base = data(blockIdx.y) + mm;
+    const auto base = as_global(data(blockIdx.y) + mm);
     const int ky = (int)(mm / nx), kx = (int)(mm - (long)ky * nx);
 
     T v[C];
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void rowcr_kernel(const BatchPtr<Ts> data, con
     T* f = e + nx;
     T* bb = f + nx;
     const int k = blockIdx.x;
-    Ts* __restrict__ row = data(blockIdx.y) + (long)k * nx;
+    const auto row = as_global(data(blockIdx.y) + (long)k * nx);
     const Ts lk = lmy[k];
     for (int j = threadIdx.x; j < nx; j += blockDim.x) {
         d[j] = (T)(-2.0 * inv_dx2 - (double)(lk * sc[j]));
@@ -397,7 +397,9 @@ static hipError_t zcr_k(const ZcrArgs& a) {
 
 template <typename Ts, typename T, int Q>
 static hipError_t zcr_g(const ZcrArgs& a) {
-    const int gm = a.gmul ? a.gmul : 1;
+    // default: 2 lanes per 64 rows (32 rows per lane) from 64 rows up: fewer
+    // registers per lane, twice the occupancy (measured faster at 128 rows)
+    const int gm = a.gmul ? a.gmul : (Q >= 6 ? 2 : 1);
     if (!zcr_gmul_ok(Q, gm)) return hipErrorInvalidValue;
     if (gm == 1) return zcr_k<Ts, T, Q, 1>(a);
     if constexpr (Q >= 2) {

@@ -123,21 +123,21 @@ void set_error_text(const char* msg) { std::snprintf(g_errbuf, sizeof g_errbuf, 
 
 static long partial_blocks(const fdmgpu_plan_t* P) {
     // residual_kernel writes residual_partials() per problem, plane_resid_dst_kernel one per z-plane
-    return std::max<long>(residual_partials(P->nx, P->ny, P->nz), P->nz);
+    return std::max<long>(residual_partials(P->nx, P->ny, P->nz, P->kind == 3), P->nz);
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-size_t work_bytes_shape(int precision, int nx, int ny, int nz, int count) {
+size_t work_bytes_shape(int kind, int precision, int nx, int ny, int nz, int count) {
     if (precision != FDMGPU_F32_REFINE64 || count <= 0) return 0;
-    const long nblk = std::max<long>(residual_partials(nx, ny, nz), nz);
+    const long nblk = std::max<long>(residual_partials(nx, ny, nz, kind == 3), nz);
     const size_t elems = (size_t)nx * ny * nz;
     return align256(sizeof(float) * elems * count) + align256(sizeof(double) * 2 * nblk * count) +
            2 * align256(sizeof(int) * count) + align256(sizeof(double) * count);
 }
 
 size_t work_bytes(const fdmgpu_plan_t* P, int count) {
-    return work_bytes_shape(P->precision, P->nx, P->ny, P->nz, count);
+    return work_bytes_shape(P->kind, P->precision, P->nx, P->ny, P->nz, count);
 }
 
 Work work_carve(const fdmgpu_plan_t* P, int count, void* mem) {
@@ -210,12 +210,16 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
     Variant d;
     d.plane = plane_capable(P) ? 1 : 0;
     out.push_back(d);   // the default first
+    const bool refine = P->precision == FDMGPU_F32_REFINE64;
     for (int gm : gms) {
         if (plane_capable(P)) {
-            Variant v;
-            v.plane = 1;
-            v.zcr_gmul = gm;
-            out.push_back(v);
+            for (int fu : (refine ? std::vector<int>{1, 0} : std::vector<int>{1})) {
+                Variant v;
+                v.plane = 1;
+                v.zcr_gmul = gm;
+                v.fuse = fu;
+                out.push_back(v);
+            }
         }
         for (int tx : (P->kind == 3 ? tiles(P->logx) : std::vector<int>{0}))
             for (int ty : tiles(P->logy)) {
@@ -232,8 +236,8 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
 
 bool variant_valid(const fdmgpu_plan_t* P, const Variant& v) {
     for (const Variant& c : plan_variants(P))
-        if (c.plane == v.plane && c.tile_x == v.tile_x && c.tile_y == v.tile_y &&
-            (c.zcr_gmul ? c.zcr_gmul : 1) == (v.zcr_gmul ? v.zcr_gmul : 1))
+        if (c.plane == v.plane && c.tile_x == v.tile_x && c.tile_y == v.tile_y && c.zcr_gmul == v.zcr_gmul &&
+            (c.fuse == v.fuse || !c.plane))
             return true;
     return false;
 }
@@ -392,7 +396,7 @@ __global__ void add_f32_to_f64(const BatchPtr<double> u, const float* __restrict
                                const int* __restrict__ done) {
     if (done && done[blockIdx.y]) return;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) u(blockIdx.y)[i] += (double)c[blockIdx.y * bs + i];
+    if (i < n) as_global(u(blockIdx.y))[i] += (double)c[blockIdx.y * bs + i];
 }
 
 // FP32 inner solves + FP64 residual/update/norm.
@@ -450,7 +454,7 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
     ra.stream = s;
     ra.f_tab = rhs_tab;
     ra.u_tab = ans_tab;
-    const bool fused = plane_capable(P) && P->var.plane;
+    const bool fused = plane_capable(P) && P->var.plane && P->var.fuse;
 
     // Fused path (cubes whose planes fit LDS): the forward DST of solve i >= 2
     // computes r = f - A u itself; convergence of the iterate after solve i is
@@ -764,14 +768,15 @@ int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_ta
 int fdmgpu_plan_variants(fdmgpu_plan plan, int max, fdmgpu_variant* out, int* count) {
     if (!plan || max < 0 || (max > 0 && !out)) return FDMGPU_ERR_INVALID_ARG;
     const std::vector<Variant> v = plan_variants(plan);
-    for (int i = 0; i < max && i < (int)v.size(); i++) out[i] = {v[i].plane, v[i].tile_x, v[i].tile_y, v[i].zcr_gmul};
+    for (int i = 0; i < max && i < (int)v.size(); i++)
+        out[i] = {v[i].plane, v[i].tile_x, v[i].tile_y, v[i].zcr_gmul, v[i].fuse};
     if (count) *count = (int)v.size();
     return FDMGPU_OK;
 }
 
 int fdmgpu_plan_set_variant(fdmgpu_plan plan, const fdmgpu_variant* v) {
     if (!plan || !v) return FDMGPU_ERR_INVALID_ARG;
-    const Variant w{v->plane, v->tile_x, v->tile_y, v->zcr_gmul};
+    const Variant w{v->plane, v->tile_x, v->tile_y, v->zcr_gmul, v->fuse};
     if (!variant_valid(plan, w)) return FDMGPU_ERR_INVALID_ARG;
     plan->var = w;
     return FDMGPU_OK;
@@ -779,7 +784,7 @@ int fdmgpu_plan_set_variant(fdmgpu_plan plan, const fdmgpu_variant* v) {
 
 int fdmgpu_plan_get_variant(fdmgpu_plan plan, fdmgpu_variant* v) {
     if (!plan || !v) return FDMGPU_ERR_INVALID_ARG;
-    *v = {plan->var.plane, plan->var.tile_x, plan->var.tile_y, plan->var.zcr_gmul};
+    *v = {plan->var.plane, plan->var.tile_x, plan->var.tile_y, plan->var.zcr_gmul, plan->var.fuse};
     return FDMGPU_OK;
 }
 

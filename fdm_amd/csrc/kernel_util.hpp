@@ -28,4 +28,14 @@ inline hipError_t allow_lds(K kernel, size_t bytes) {
 
 constexpr size_t kLdsBytes = 160 * 1024;
 
+// Waves per SIMD that the LDS footprint allows for blocks of `threads`
+// threads using `lds` bytes each (4 SIMDs per CU), capped at 4: the register
+// budget the compiler is held to (512 / waves VGPRs), so that registers never
+// cut the occupancy below what LDS allows.
+constexpr int lds_waves_per_simd(size_t lds, int threads) {
+    const size_t blocks = lds ? kLdsBytes / lds : 8;
+    const int waves = (int)(blocks * (size_t)((threads + 63) / 64) / 4);
+    return waves < 1 ? 1 : (waves > 4 ? 4 : waves);
+}
+
 }  // namespace fdmgpu

@@ -18,6 +18,17 @@ struct BatchPtr {
     }
 };
 
+// The same address as a global-memory pointer: loads and stores through it are
+// global_* instructions, which cannot alias LDS, so the compiler can keep many
+// in flight (through a generic pointer they are flat_* and every one is
+// waited for before the next LDS access).
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ global_ptr<T> as_global(T* p) {
+    return (global_ptr<T>)p;
+}
+
 // Pencil transform kinds (see pencil_pairs in dst_device.hpp): DST-I on
 // Dirichlet axes, half-complex real DFT forward / inverse on periodic axes.
 enum : int { kDst = 0, kRdftFwd = 1, kRdftInv = 2 };

@@ -34,6 +34,8 @@ struct Variant {
     int tile_x = 0;     // x tile kernel: rows per block (16/32/64)
     int tile_y = 0;     // y tile kernel: columns per block
     int zcr_gmul = 0;   // z cyclic reduction: lanes-per-system multiplier (1/2)
+    int fuse = 1;       // FP32+refine on the plane path: 1 = residual fused into the forward
+                        // plane transform, 0 = separate residual pass writing the FP32 rhs
 };
 
 }  // namespace fdmgpu
@@ -88,7 +90,7 @@ namespace fdmgpu {
 // Device bytes of the scratch a solve of `count` problems needs (each array
 // 256-byte aligned), for a plan or for a shape and precision.
 size_t work_bytes(const fdmgpu_plan_t* P, int count);
-size_t work_bytes_shape(int precision, int nx, int ny, int nz, int count);
+size_t work_bytes_shape(int kind, int precision, int nx, int ny, int nz, int count);
 // A Work over caller-provided memory of work_bytes(P, count) bytes (never freed
 // by work_free's owner: do not pass it to work_free).
 Work work_carve(const fdmgpu_plan_t* P, int count, void* mem);

@@ -33,12 +33,35 @@ struct PlaneCfg {
     static constexpr size_t lds_bytes() {
         return sizeof(T) * (size_t)N * P;
     }
+    template <typename T>
+    static constexpr int waves() {
+        return lds_waves_per_simd(lds_bytes<T>(), NT);
+    }
 };
+
+// Global <-> LDS sweep over an n x n plane: thread tid owns column
+// x = tid % N and rows r0, r0 + R, ... (r0 = tid / N, R = NT / N), so the
+// global accesses of a wavefront are consecutive x and the row-dependent part
+// of every address is a per-iteration constant (no per-element division).
+template <int LOGN, int NT, int NN, int UNROLL = 64, typename F>
+__device__ __forceinline__ void plane_sweep(const int tid, F&& f) {
+    constexpr int N = 1 << LOGN, R = NT / N, ITER = (NN + R - 1) / R;
+    static_assert(NT % N == 0, "a block covers whole rows");
+    const int x = tid & (N - 1), r0 = tid >> LOGN;
+    if (x < NN) {
+#pragma unroll UNROLL
+        for (int i = 0; i < ITER; ++i) {
+            const int y = r0 + R * i;
+            if (R * i + R <= NN || y < NN) f(y, x);
+        }
+    }
+}
 
 // KIND (dst_device.hpp): kDst for Dirichlet planes of (N-1)^2 points,
 // kRdftFwd / kRdftInv for periodic planes of N^2 points.
 template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND>
-__global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_dst_kernel(
+__global__ __launch_bounds__(PlaneCfg<LOGN>::NT)
+__attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void plane_dst_kernel(
     const BatchPtr<const Tin> in, const BatchPtr<Tout> out, const T sx, const T sy, const T* __restrict__ tables,
     const int* __restrict__ done) {
     using C = PlaneCfg<LOGN>;
@@ -52,8 +75,8 @@ __global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_dst_kernel(
     T* lds = reinterpret_cast<T*>(smem_raw);
     const int tid = threadIdx.x;
     const long plane_off = (long)blockIdx.x * n * n;
-    const Tin* __restrict__ src = in(blockIdx.y) + plane_off;
-    Tout* __restrict__ dst = out(blockIdx.y) + plane_off;
+    const auto src = as_global(in(blockIdx.y) + plane_off);
+    const auto dst = as_global(out(blockIdx.y) + plane_off);
 
     DstLane<T, LOGN> L;
     L.init(tables, tid & (S::N1 - 1));
@@ -63,21 +86,17 @@ __global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_dst_kernel(
             lds[LX::at(i, 0)] = T(0);   // dummy column 0
         }
     }
-    for (int idx = tid; idx < n * n; idx += NT) {
-        const int y = idx / n, x = idx - y * n;
-        lds[LX::at(y + o, x + o)] = (T)src[idx];
-    }
+    plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) { lds[LX::at(y + o, x + o)] = (T)src[y * n + x]; });
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, N / 2, LX, NT>(lds, L, sx, tid);   // pencils = rows    (x)
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, N / 2, LY, NT>(lds, L, sy, tid);   // pencils = columns (y)
     __syncthreads();
-    for (int idx = tid; idx < n * n; idx += NT) {
-        const int y = idx / n, x = idx - y * n;
+    plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) {
         const T v = lds[LX::at(IO::out(y), IO::out(x))];
-        if constexpr (ACCUM) dst[idx] += (Tout)v;
-        else dst[idx] = (Tout)v;
-    }
+        if constexpr (ACCUM) dst[y * n + x] += (Tout)v;
+        else dst[y * n + x] = (Tout)v;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -97,7 +116,7 @@ __device__ __forceinline__ int xcd_slab_plane(int bid, int nplanes) {
 }
 
 template <typename Tu>
-__device__ __forceinline__ double residual_at(const double* __restrict__ f, const Tu* __restrict__ u, const long c,
+__device__ __forceinline__ double residual_at(const global_ptr<const double> f, const global_ptr<const Tu> u, const long c,
                                               const int x, const int y, const int z, const int nx, const int ny,
                                               const int nz, const double ix, const double iy, const double iz,
                                               double& fv) {
@@ -137,7 +156,8 @@ __device__ __forceinline__ void block_sum2_to(double rr, double ff, double* __re
 }
 
 template <typename Tu, int LOGN>
-__global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_resid_dst_kernel(
+__global__ __launch_bounds__(PlaneCfg<LOGN>::NT)
+__attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<float>()))) void plane_resid_dst_kernel(
     const BatchPtr<const double> f, const BatchPtr<const Tu> u, float* __restrict__ out,
     double* __restrict__ partial, const long bs, const int nplanes, const double ix, const double iy,
     const double iz, const float sx, const float sy, const float* __restrict__ tables,
@@ -161,17 +181,17 @@ __global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_resid_dst_kernel(
         lds[LX::at(0, i)] = 0.f;
         lds[LX::at(i, 0)] = 0.f;
     }
-    const double* __restrict__ fp = f(p);
-    const Tu* __restrict__ up = u(p);
+    const auto fp = as_global(f(p));
+    const auto up = as_global(u(p));
     double rr = 0.0, ff = 0.0;
-    for (int idx = tid; idx < n * n; idx += NT) {
-        const int y = idx / n, x = idx - y * n;
+    plane_sweep<LOGN, NT, n, 2>(tid, [&](int y, int x) {   // 8 loads per point: unroll little (registers)
         double fv;
-        const double r = residual_at<Tu>(fp, up, (long)z * n * n + idx, x, y, z, n, n, nplanes, ix, iy, iz, fv);
+        const double r =
+            residual_at<Tu>(fp, up, (long)z * n * n + y * n + x, x, y, z, n, n, nplanes, ix, iy, iz, fv);
         rr += r * r;
         ff += fv * fv;
         lds[LX::at(y + 1, x + 1)] = (float)r;
-    }
+    });
     block_sum2_to<NT>(rr, ff, partial + ((long)p * nplanes + z) * 2, red);
     __syncthreads();
     dst_pairs<float, LOGN, N / 2, LX, NT>(lds, L, sx, tid);
@@ -179,10 +199,7 @@ __global__ __launch_bounds__(PlaneCfg<LOGN>::NT) void plane_resid_dst_kernel(
     dst_pairs<float, LOGN, N / 2, LY, NT>(lds, L, sy, tid);
     __syncthreads();
     float* dst = out + p * bs + (long)z * n * n;
-    for (int idx = tid; idx < n * n; idx += NT) {
-        const int y = idx / n, x = idx - y * n;
-        dst[idx] = lds[LX::at(S::sigma(y + 1), S::sigma(x + 1))];
-    }
+    plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) { dst[y * n + x] = lds[LX::at(S::sigma(y + 1), S::sigma(x + 1))]; });
 }
 
 template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND>

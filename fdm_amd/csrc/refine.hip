@@ -11,6 +11,8 @@
 // residual and iteration count do not depend on what else is in the batch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "refine.hpp"
 
 namespace fdmgpu {
@@ -51,8 +53,8 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
     __shared__ double sh[2 * kResBlock];
     const int p = blockIdx.y;
     if (done && done[p]) return;
-    const double* __restrict__ fp = f(p);
-    const double* __restrict__ up = u(p);
+    const auto fp = as_global(f(p));
+    const auto up = as_global(u(p));
     float* rp = r32 ? r32 + p * bs : nullptr;
     const long pl = (long)nx * ny;
     const int nblk = gridDim.x;
@@ -95,6 +97,58 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
     }
 }
 
+// 3-D residual marching along z: a block of 64 x 4 threads covers
+// x in [x0, x0+64), y in [y0, y0+4) and walks z; each thread keeps u at
+// z-1, z, z+1 of its column in registers, so every u value is read from HBM
+// once and the x / y neighbours come from L1/L2 (the neighbouring threads'
+// loads of the same plane).  partial[p][blk] as residual_kernel, blk =
+// blockIdx.y * gridDim.x + blockIdx.x.
+constexpr int kZX = 64, kZY = 4;
+
+template <bool WRITE>
+__global__ __launch_bounds__(kZX* kZY) void residual_z_kernel(
+    const BatchPtr<const double> f, const BatchPtr<const double> u, float* __restrict__ r32,
+    double* __restrict__ partial, const int nx, const int ny, const int nz, const double ix, const double iy,
+    const double iz, const long bs, const int* __restrict__ done) {
+    __shared__ double sh[2 * kResBlock];
+    const int p = blockIdx.z;
+    if (done && done[p]) return;
+    const auto fp = as_global(f(p));
+    const auto up = as_global(u(p));
+    const int x = blockIdx.x * kZX + (threadIdx.x & (kZX - 1));
+    const int y = blockIdx.y * kZY + (threadIdx.x / kZX);
+    const bool valid = x < nx && y < ny;
+    const long pl = (long)nx * ny;
+    const long c0 = (long)y * nx + x;
+    const bool hx0 = x > 0, hx1 = x < nx - 1, hy0 = y > 0, hy1 = y < ny - 1;
+    double rr = 0.0, ff = 0.0;
+    if (valid) {
+        double um = 0.0, uc = up[c0];
+#pragma unroll 4
+        for (int z = 0; z < nz; ++z) {
+            const long c = z * pl + c0;
+            const double un = z + 1 < nz ? up[c + pl] : 0.0;
+            const double xm = hx0 ? up[c - 1] : 0.0, xp = hx1 ? up[c + 1] : 0.0;
+            const double ym = hy0 ? up[c - nx] : 0.0, yp = hy1 ? up[c + nx] : 0.0;
+            const double au = (xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy + (um - 2.0 * uc + un) * iz;
+            const double fv = fp[c];
+            const double r = fv - au;
+            if constexpr (WRITE) r32[p * bs + c] = (float)r;
+            rr += r * r;
+            ff += fv * fv;
+            um = uc;
+            uc = un;
+        }
+    }
+    block_sum2<double>(rr, ff, sh);
+    if (threadIdx.x == 0) {
+        const int nblk = gridDim.x * gridDim.y;
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        partial[((long)p * nblk + b) * 2 + 0] = rr;
+        partial[((long)p * nblk + b) * 2 + 1] = ff;
+    }
+}
+
 // One block per problem: fixed-order sum of the partials, then the flag.
 __global__ __launch_bounds__(kResBlock) void check_kernel(const double* __restrict__ partial, const int nblk,
                                                          const double tol, int* __restrict__ done,
@@ -128,12 +182,17 @@ __global__ void fill_int_kernel(int* __restrict__ a, int* __restrict__ b, const 
 constexpr long kResMaxBlocks = 1024;   // per problem
 
 hipError_t launch_residual(const ResidualArgs& a) {
-    const unsigned nblk = (unsigned)residual_partials(a.nx, a.ny, a.nz);
+    const unsigned nblk = (unsigned)residual_partials(a.nx, a.ny, a.nz, a.nz > 1 || a.three);
     dim3 grid(nblk, a.batch);
     const BatchPtr<const double> f{a.f, a.f_tab, a.bs}, u{a.u, a.u_tab, a.bs};
     if (a.nz > 1 || a.three) {
-        hipLaunchKernelGGL((residual_kernel<true, false>), grid, dim3(kResBlock), 0, a.stream, f, u, a.r32,
-                           a.partial, a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, nullptr, nullptr, nullptr, a.bs, a.done);
+        dim3 zg((a.nx + kZX - 1) / kZX, (a.ny + kZY - 1) / kZY, a.batch);
+        if (a.r32)
+            hipLaunchKernelGGL((residual_z_kernel<true>), zg, dim3(kZX * kZY), 0, a.stream, f, u, a.r32, a.partial,
+                               a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.done);
+        else
+            hipLaunchKernelGGL((residual_z_kernel<false>), zg, dim3(kZX * kZY), 0, a.stream, f, u, a.r32, a.partial,
+                               a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.done);
     } else if (a.sy) {
         hipLaunchKernelGGL((residual_kernel<false, true>), grid, dim3(kResBlock), 0, a.stream, f, u, a.r32,
                            a.partial, a.nx, a.ny, 1, a.ix, a.iy, 0.0, a.sy, a.lx, a.ux, a.bs, a.done);
@@ -154,10 +213,10 @@ hipError_t launch_check(const double* partial, int nblk, double tol, int* done, 
     return hipGetLastError();
 }
 
-// chunks per problem: about 4 rows each, a multiple of 8 (one slab per XCD),
-// at most kResMaxBlocks
-long residual_partials(int nx, int ny, int nz) {
-    (void)nx;
+// partials per problem: 3-D, one per residual_z_kernel block; 2-D, chunks of
+// about 4 rows, a multiple of 8 (one slab per XCD), at most kResMaxBlocks
+long residual_partials(int nx, int ny, int nz, bool three) {
+    if (three) return (long)((nx + kZX - 1) / kZX) * ((ny + kZY - 1) / kZY);
     const long rows = (long)ny * nz;
     long b = (rows + 3) / 4;
     b = (b + 7) / 8 * 8;

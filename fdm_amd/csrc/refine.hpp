@@ -34,7 +34,8 @@ hipError_t launch_residual(const ResidualArgs& a);
 // norm check over `nblk` partials per problem (fixed order); see refine.hip
 hipError_t launch_check(const double* partial, int nblk, double tol, int* done, int* iters, double* resid, int it,
                         int batch, hipStream_t s);
-long residual_partials(int nx, int ny, int nz);
+// partials per problem of launch_residual (three: the 7-point cube operator)
+long residual_partials(int nx, int ny, int nz, bool three);
 // a[i] = b[i] = v for i < n (b may be null); a kernel, so it can be captured in a graph
 hipError_t launch_fill_int(int* a, int* b, int n, int v, hipStream_t s);
 

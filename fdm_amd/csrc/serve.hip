@@ -90,7 +90,8 @@ int check_problem(const fdmgpu_problem& p) {
 
 // Scratch bytes of a batch: the refinement Work plus the pointer tables.
 size_t batch_bytes(const Key& k, int count) {
-    return work_bytes_shape(k.precision, k.nx, k.ny, k.nz, count) + ((2 * sizeof(void*) * count + 255) & ~(size_t)255);
+    return work_bytes_shape(k.kind, k.precision, k.nx, k.ny, k.nz, count) +
+           ((2 * sizeof(void*) * count + 255) & ~(size_t)255);
 }
 
 long problem_elems(const Key& k) { return (long)k.nx * k.ny * k.nz; }
@@ -108,7 +109,7 @@ std::string tune_key(int kind, int precision, int periodic, int nx, int ny, int 
 }
 
 // Cache file format (written by fdmgpu_tune): one entry per line,
-//   "<key>": {"plane": P, "tile_x": X, "tile_y": Y, "zcr_gmul": G, "ms": T},
+//   "<key>": {"plane": P, "tile_x": X, "tile_y": Y, "zcr_gmul": G, "fuse": F, "ms": T},
 void tune_read_file(const char* path) {
     FILE* f = std::fopen(path, "r");
     if (!f) return;
@@ -117,8 +118,10 @@ void tune_read_file(const char* path) {
         char key[128];
         Variant v;
         double ms = 0;
-        if (std::sscanf(line, " \"%127[^\"]\": {\"plane\": %d, \"tile_x\": %d, \"tile_y\": %d, \"zcr_gmul\": %d, \"ms\": %lf",
-                        key, &v.plane, &v.tile_x, &v.tile_y, &v.zcr_gmul, &ms) == 6)
+        if (std::sscanf(line,
+                        " \"%127[^\"]\": {\"plane\": %d, \"tile_x\": %d, \"tile_y\": %d, \"zcr_gmul\": %d, "
+                        "\"fuse\": %d, \"ms\": %lf",
+                        key, &v.plane, &v.tile_x, &v.tile_y, &v.zcr_gmul, &v.fuse, &ms) == 7)
             g_tune[key] = v;
     }
     std::fclose(f);
@@ -846,8 +849,10 @@ int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char*
         size_t i = 0;
         for (auto& kv : all) {
             const Variant& v = kv.second.first;
-            std::fprintf(f, "  \"%s\": {\"plane\": %d, \"tile_x\": %d, \"tile_y\": %d, \"zcr_gmul\": %d, \"ms\": %.6f}%s\n",
-                         kv.first.c_str(), v.plane, v.tile_x, v.tile_y, v.zcr_gmul, kv.second.second,
+            std::fprintf(f,
+                         "  \"%s\": {\"plane\": %d, \"tile_x\": %d, \"tile_y\": %d, \"zcr_gmul\": %d, "
+                         "\"fuse\": %d, \"ms\": %.6f}%s\n",
+                         kv.first.c_str(), v.plane, v.tile_x, v.tile_y, v.zcr_gmul, v.fuse, kv.second.second,
                          ++i < all.size() ? "," : "");
         }
         std::fprintf(f, "}\n");

@@ -101,10 +101,13 @@ int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_ta
  * plane = 1 runs square planes that fit LDS through the plane kernel (0: x
  * and y tile kernels); tile_x / tile_y = pencils per block of the tile
  * kernels (16, 32, 64; 0 = default); zcr_gmul = lanes per z system
- * multiplier (1, 2; 0 = 1).  Every variant solves the same system; results
+ * multiplier (1, 2; 0 = 2 from 64 rows, else 1); fuse (FP32+refine plane
+ * path) = FP64 residual fused into the next forward transform (1) or a
+ * separate residual pass (0).  Every variant solves the same system; results
  * differ at rounding level.  No reference counterpart. */
 typedef struct fdmgpu_variant {
     int plane, tile_x, tile_y, zcr_gmul;
+    int fuse; /* FP32+refine on the plane path: 1 = residual fused into the forward transform */
 } fdmgpu_variant;
 int fdmgpu_plan_variants(fdmgpu_plan plan, int max, fdmgpu_variant* out, int* count);
 int fdmgpu_plan_set_variant(fdmgpu_plan plan, const fdmgpu_variant* variant);
