@@ -18,6 +18,9 @@ Legs (each can be skipped):
         5% 255^3, FP32+refine, at 0.5x / 0.8x / 1.0x of the measured
         saturation rate; p50/p99 latency from the scheduled arrival to the
         completion; FP64 one-at-a-time solves of the same mix beside it.
+  c5    NSCube lid-driven cavity 127^3 (Re 250, dt 0.01, u0 1), 1000 steps
+        with the FP32+refine projection; steps/s; compared with the CPU
+        oracle's FP64 NSCube at step 100.
   cpu   the reference's LaplCube<double> on the host cores (bounded sample).
 
 Prints ONE JSON line.
@@ -420,6 +423,50 @@ def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 32
     return out
 
 
+def leg_c5(n: int = 127, steps: int = 1000, check_at: int = 100, cpu_check: bool = True):
+    """NSCube lid-driven cavity n^3 on [-pi,pi]^3, Re 250, dt 0.01, u0 1, from
+    rest; pressure projection = the FP32+refine solve.  Compared with the CPU
+    oracle's FP64 NSCube at step `check_at`, then timed to `steps`."""
+    import fdm_amd
+    from fdm_amd import hip
+    import oracle as O
+    kw = dict(u0=1.0, Re=250.0, dt=0.01)
+    g = fdm_amd.NSCube(n, n, n, solver="refine", **kw)
+    out = {"workload": f"NSCube lid-driven cavity {n}^3 on [-pi,pi]^3, Re 250, dt 0.01, u0 1, from rest; "
+                       "pressure solve FP32+refine to 1e-10", "steps": steps}
+    st = hip.Stream()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(st)
+    g.step(check_at, st)
+    e1.record(st)
+    e1.synchronize()
+    first_ms = e0.elapsed_ms(e1)
+    if cpu_check:
+        o = O.NsCube(n, n, n, **kw)
+        t = time.perf_counter()
+        o.step(check_at)
+        cpu_s = time.perf_counter() - t
+        out["compare_at_step"] = check_at
+        out["max_rel_diff_vs_cpu_fp64"] = {f: float(np.abs(g.get(f) - o.get(f)).max() / np.abs(o.get(f)).max())
+                                           for f in ("u", "v", "w", "p")}
+        out["cpu_oracle_steps_per_s"] = round(check_at / cpu_s, 2)
+        del o
+    rest = steps - check_at
+    e0.record(st)
+    g.step(rest, st)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    s = g.solver_stats()
+    u = g.get("u")
+    out.update({"steps_per_s": round(rest / (ms / 1e3), 1), "ms_per_step": round(ms / rest, 4),
+                "first_steps_ms_per_step": round(first_ms / check_at, 4), "last_solve_rel_residual": s.rel_residual,
+                "last_solve_iterations": s.iterations, "u_max": float(np.abs(u[1:n + 1]).max()),
+                "finite": bool(np.isfinite(u).all())})
+    g.close()
+    return out
+
+
 def cpu_baseline(n: int, f0: np.ndarray, budget_s: float = 15.0):
     """The reference's own LaplCube<double,false> (oracle/_ref, built from the reference
     sources) when present, else the oracle restatement; OpenMP over the host cores."""
@@ -474,8 +521,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mode", choices=("both", "refine", "f64"), default="both",
                     help="which GPU legs to run (profiling runs use one)")
-    ap.add_argument("--legs", default="head,c2,c3,c4,cpu",
-                    help="comma list of legs: head (the value; always run), c2, c3, c4, cpu")
+    ap.add_argument("--legs", default="head,c2,c3,c4,c5,cpu",
+                    help="comma list of legs: head (the value; always run), c2, c3, c4, c5, cpu")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving rate")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (calibrates the HBM "
@@ -580,6 +627,9 @@ def main():
         out["serve"] = leg_c4(duration_s=args.serve_seconds)
         out["p50_ms"] = [r["p50_ms"] for r in out["serve"]["rates"]]
         out["p99_ms"] = [r["p99_ms"] for r in out["serve"]["rates"]]
+    if "c5" in legs:
+        progress("c5: NSCube 127^3 cavity, 1000 steps (CPU oracle check at step 100)")
+        out["nscube"] = leg_c5(cpu_check="cpu" in legs)
     if "cpu" in legs:
         progress("cpu: reference LaplCube<double> on the host cores")
         cb, ans_cpu = cpu_baseline(n, fcpu, args.cpu_budget)

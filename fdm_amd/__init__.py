@@ -36,7 +36,8 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_rect_set_scales", "fd
            "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_group_create",
            "fdmgpu_group_solve", "fdmgpu_group_result", "fdmgpu_group_destroy", "fdmgpu_server_create",
            "fdmgpu_server_submit", "fdmgpu_server_set_refine", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
-           "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_destroy", "fdmgpu_strerror",
+           "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_ns_create", "fdmgpu_ns_step", "fdmgpu_ns_get", "fdmgpu_ns_set", "fdmgpu_ns_info",
+           "fdmgpu_ns_solver", "fdmgpu_ns_destroy", "fdmgpu_destroy", "fdmgpu_strerror",
            "fdmgpu_version")
 
 # kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
@@ -111,6 +112,7 @@ def _lib_handle():
         L.fdmgpu_plan_footprint.argtypes = [vp, i, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
         L.fdmgpu_set_timing.argtypes = [vp, i]
         L.fdmgpu_get_timing.argtypes = [vp, i, C.POINTER(d), C.POINTER(i)]
+        ull = C.c_ulonglong
         L.fdmgpu_solve_table_device.argtypes = [vp, i, vp, vp, vp]
         L.fdmgpu_plan_variants.argtypes = [vp, i, C.POINTER(Variant), C.POINTER(i)]
         L.fdmgpu_plan_set_variant.argtypes = [vp, C.POINTER(Variant)]
@@ -129,6 +131,13 @@ def _lib_handle():
         L.fdmgpu_server_forget.argtypes = [vp, ull]
         L.fdmgpu_server_destroy.argtypes = [vp]
         L.fdmgpu_tune.argtypes = [i, C.POINTER(Problem), i, C.c_char_p]
+        L.fdmgpu_ns_create.argtypes = [C.POINTER(vp), i, i, i, i] + [d] * 9
+        L.fdmgpu_ns_step.argtypes = [vp, i, vp]
+        L.fdmgpu_ns_get.argtypes = [vp, i, vp, ull]
+        L.fdmgpu_ns_set.argtypes = [vp, i, vp, ull]
+        L.fdmgpu_ns_info.argtypes = [vp, i, C.POINTER(ull), C.POINTER(vp)]
+        L.fdmgpu_ns_solver.argtypes = [vp, C.POINTER(vp)]
+        L.fdmgpu_ns_destroy.argtypes = [vp]
         L.fdmgpu_destroy.argtypes = [vp]
         L.fdmgpu_strerror.argtypes = [i]
         L.fdmgpu_strerror.restype = C.c_char_p
@@ -351,6 +360,68 @@ class LaplRect(_Plan):
 
         _check(_lib_handle().fdmgpu_rect_set_scales(self._h, ptr(lm_y_scale), ptr(L_scale), ptr(U_scale)),
                "LaplRect.set_scales")
+
+
+NS_FIELDS = ("u", "v", "w", "p", "x", "F", "G", "H", "RHS")
+
+
+def ns_shapes(nx, ny, nz):
+    """Field shapes (z, y, x) with the reference's index ranges (src/ns_cube.h:66-75)."""
+    return {"u": (nz + 2, ny + 2, nx + 3), "v": (nz + 2, ny + 3, nx + 2), "w": (nz + 3, ny + 2, nx + 2),
+            "p": (nz + 2, ny + 2, nx + 2), "x": (nz, ny, nx), "F": (nz, ny, nx + 1), "G": (nz, ny + 1, nx),
+            "H": (nz + 1, ny, nx), "RHS": (nz, ny, nx)}
+
+
+class NSCube:
+    """NSCube<double,check>(Config) (reference src/ns_cube.h:12-78) on the GPU: the
+    lid-driven cavity whose pressure projection is the fast Poisson solve.
+    solver: "refine" (FP32 transforms + FP64 refinement) or "f64"."""
+
+    def __init__(self, nx=32, ny=32, nz=32, x1=-np.pi, x2=np.pi, y1=-np.pi, y2=np.pi, z1=-np.pi, z2=np.pi,
+                 u0=1.0, Re=1.0, dt=0.001, solver="refine"):
+        h = C.c_void_p()
+        _check(_lib_handle().fdmgpu_ns_create(C.byref(h), _prec(solver), nx, ny, nz, x1, x2, y1, y2, z1, z2,
+                                              u0, Re, dt), "NSCube")
+        self._h = h
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.dx, self.dy, self.dz = (x2 - x1) / nx, (y2 - y1) / ny, (z2 - z1) / nz
+        self.shapes = ns_shapes(nx, ny, nz)
+        self.time_index = 0
+
+    def step(self, count: int = 1, stream=None):
+        _check(_lib_handle().fdmgpu_ns_step(self._h, count, _stream_handle(stream)), "NSCube.step")
+        self.time_index += count
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.empty(self.shapes[name], np.float64)
+        _check(_lib_handle().fdmgpu_ns_get(self._h, NS_FIELDS.index(name), out.ctypes.data_as(C.c_void_p),
+                                           out.size), "NSCube.get")
+        return out
+
+    def set(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.shape != self.shapes[name]:
+            raise ValueError(f"{name} has shape {self.shapes[name]}")
+        _check(_lib_handle().fdmgpu_ns_set(self._h, NS_FIELDS.index(name), a.ctypes.data_as(C.c_void_p), a.size),
+               "NSCube.set")
+
+    def solver_stats(self) -> Stats:
+        plan = C.c_void_p()
+        _check(_lib_handle().fdmgpu_ns_solver(self._h, C.byref(plan)), "NSCube.solver")
+        st = Stats()
+        _check(_lib_handle().fdmgpu_get_stats(plan, C.byref(st)), "get_stats")
+        return st
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib_handle().fdmgpu_ns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version() -> int:

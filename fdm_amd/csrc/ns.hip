@@ -1,0 +1,398 @@
+// NSCube time step on the GPU (C ABI in include/fdmgpu.h: fdmgpu_ns_*).
+//
+// The reference's lid-driven cavity (src/ns_cube.h:12-78,
+// src/ns_cube.cpp:27-277): staggered MAC grid, explicit advection + diffusion
+// (F, G, H), pressure Poisson projection through LaplCube, velocity update.
+// Every array lives in HBM with the reference's index ranges (x fastest):
+//   u  i in [0,nz+1], k in [0,ny+1], j in [-1,nx+1]
+//   v  i in [0,nz+1], k in [-1,ny+1], j in [0,nx+1]
+//   w  i in [-1,nz+1], k in [0,ny+1], j in [0,nx+1]
+//   p  [0,n+1]^3;  x, RHS [1,n]^3;  F j in [0,nx];  G k in [0,ny];  H i in [0,nz]
+// A step is 6 kernels plus the Poisson solve on one stream; from the third
+// step on it is replayed from a hipGraph captured from one step.
+// Arithmetic is FP64 except inside an FP32+refine Poisson solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using namespace fdmgpu;
+
+namespace {
+
+struct Grid {
+    int nx, ny, nz;
+    double dx, dy, dz, U0, Re, dt;
+    // element offsets of (i, k, j) in each array
+    __device__ long u(int i, int k, int j) const { return ((long)i * (ny + 2) + k) * (nx + 3) + (j + 1); }
+    __device__ long v(int i, int k, int j) const { return ((long)i * (ny + 3) + (k + 1)) * (nx + 2) + j; }
+    __device__ long w(int i, int k, int j) const { return ((long)(i + 1) * (ny + 2) + k) * (nx + 2) + j; }
+    __device__ long p(int i, int k, int j) const { return ((long)i * (ny + 2) + k) * (nx + 2) + j; }
+    __device__ long x(int i, int k, int j) const { return ((long)(i - 1) * ny + (k - 1)) * nx + (j - 1); }
+    __device__ long F(int i, int k, int j) const { return ((long)(i - 1) * ny + (k - 1)) * (nx + 1) + j; }
+    __device__ long G(int i, int k, int j) const { return ((long)(i - 1) * (ny + 1) + k) * nx + (j - 1); }
+    __device__ long H(int i, int k, int j) const { return ((long)i * ny + (k - 1)) * nx + (j - 1); }
+};
+
+struct Fields {
+    double *u, *v, *w, *p, *x, *F, *G, *H, *RHS;
+};
+
+__device__ __forceinline__ double sq(double a) { return a * a; }
+
+// lid: u[nz+1][k][j] = 2 U0 - u[nz][k][j], k in [0, ny+1], j in [-1, nz+1]
+// (src/ns_cube.cpp:66-72; the x bound is nz+1 in the reference)
+__global__ void ns_lid_kernel(Grid g, double* __restrict__ u) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x - 1;
+    const int k = blockIdx.y;
+    if (j > g.nz + 1) return;
+    const auto U = as_global(u);
+    U[g.u(g.nz + 1, k, j)] = 2 * g.U0 - U[g.u(g.nz, k, j)];
+}
+
+// ghost layers (src/ns_cube.cpp:74-96): blockIdx.z = 0 u in x, 1 v in y, 2 w in z
+__global__ void ns_ghost_kernel(Grid g, double* __restrict__ u, double* __restrict__ v, double* __restrict__ w) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;   // fastest index of the face
+    const int b = blockIdx.y;
+    if (blockIdx.z == 0) {   // i = b in [0, nz+1], k = a in [0, ny+1]
+        if (a > g.ny + 1 || b > g.nz + 1) return;
+        const auto U = as_global(u);
+        U[g.u(b, a, -1)] = U[g.u(b, a, 1)];
+        U[g.u(b, a, g.nx + 1)] = U[g.u(b, a, g.nx - 1)];
+    } else if (blockIdx.z == 1) {   // i = b in [0, nz+1], j = a in [0, nx+1]
+        if (a > g.nx + 1 || b > g.nz + 1) return;
+        const auto V = as_global(v);
+        V[g.v(b, -1, a)] = V[g.v(b, 1, a)];
+        V[g.v(b, g.ny + 1, a)] = V[g.v(b, g.ny - 1, a)];
+    } else {   // k = b in [0, ny+1], j = a in [0, nx+1]
+        if (a > g.nx + 1 || b > g.ny + 1) return;
+        const auto W = as_global(w);
+        W[g.w(-1, b, a)] = W[g.w(1, b, a)];
+        W[g.w(g.nz + 1, b, a)] = W[g.w(g.nz - 1, b, a)];
+    }
+}
+
+// pressure on the faces (src/ns_cube.cpp:98-127): blockIdx.z = 0 x-faces
+// (i, k), 1 y-faces (i, j), 2 z-faces (k, j); face index b + 1, a + 1
+__global__ void ns_pface_kernel(Grid g, const double* __restrict__ u, const double* __restrict__ v,
+                                const double* __restrict__ w, double* __restrict__ p) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int b = blockIdx.y + 1;
+    const auto P = as_global(p);
+    if (blockIdx.z == 0) {   // i = b, k = a
+        if (a > g.ny || b > g.nz) return;
+        const auto U = as_global(u);
+        const int i = b, k = a, nx = g.nx;
+        P[g.p(i, k, 0)] = P[g.p(i, k, 1)] - (U[g.u(i, k, 1)] - 2 * U[g.u(i, k, 0)] + U[g.u(i, k, -1)]) / g.Re / g.dx;
+        P[g.p(i, k, nx + 1)] =
+            P[g.p(i, k, nx)] - (U[g.u(i, k, nx + 1)] - 2 * U[g.u(i, k, nx)] + U[g.u(i, k, nx - 1)]) / g.Re / g.dx;
+    } else if (blockIdx.z == 1) {   // i = b, j = a
+        if (a > g.nx || b > g.nz) return;
+        const auto V = as_global(v);
+        const int i = b, j = a, ny = g.ny;
+        P[g.p(i, 0, j)] = P[g.p(i, 1, j)] - (V[g.v(i, 1, j)] - 2 * V[g.v(i, 0, j)] + V[g.v(i, -1, j)]) / g.Re / g.dy;
+        P[g.p(i, ny + 1, j)] =
+            P[g.p(i, ny, j)] - (V[g.v(i, ny + 1, j)] - 2 * V[g.v(i, ny, j)] + V[g.v(i, ny - 1, j)]) / g.Re / g.dy;
+    } else {   // k = b, j = a
+        if (a > g.nx || b > g.ny) return;
+        const auto W = as_global(w);
+        const int k = b, j = a, nz = g.nz;
+        P[g.p(0, k, j)] = P[g.p(1, k, j)] - (W[g.w(1, k, j)] - 2 * W[g.w(0, k, j)] + W[g.w(-1, k, j)]) / g.Re / g.dz;
+        P[g.p(nz + 1, k, j)] =
+            P[g.p(nz, k, j)] - (W[g.w(nz + 1, k, j)] - 2 * W[g.w(nz, k, j)] + W[g.w(nz - 1, k, j)]) / g.Re / g.dz;
+    }
+}
+
+// F, G, H over the union box i in [0,nz], k in [0,ny], j in [0,nx]
+// (src/ns_cube.cpp:131-201)
+__global__ void ns_fgh_kernel(Grid g, Fields f) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (j > g.nx || k > g.ny) return;
+    const auto U = as_global((const double*)f.u);
+    const auto V = as_global((const double*)f.v);
+    const auto W = as_global((const double*)f.w);
+    const double Re = g.Re, dt = g.dt, dx = g.dx, dy = g.dy, dz = g.dz;
+    const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+    if (i >= 1 && k >= 1) {
+        const double uc = U[g.u(i, k, j)];
+        as_global(f.F)[g.F(i, k, j)] =
+            uc + dt * ((U[g.u(i, k, j + 1)] - 2 * uc + U[g.u(i, k, j - 1)]) / Re / dx2 +
+                       (U[g.u(i, k + 1, j)] - 2 * uc + U[g.u(i, k - 1, j)]) / Re / dy2 +
+                       (U[g.u(i + 1, k, j)] - 2 * uc + U[g.u(i - 1, k, j)]) / Re / dz2 -
+                       (sq(0.5 * (uc + U[g.u(i, k, j + 1)])) - sq(0.5 * (U[g.u(i, k, j - 1)] + uc))) / dx -
+                       0.25 * ((uc + U[g.u(i, k + 1, j)]) * (V[g.v(i, k, j + 1)] + V[g.v(i, k, j)]) -
+                               (U[g.u(i, k - 1, j)] + uc) * (V[g.v(i, k - 1, j + 1)] + V[g.v(i, k - 1, j)])) / dy -
+                       0.25 * ((uc + U[g.u(i + 1, k, j)]) * (W[g.w(i, k, j + 1)] + W[g.w(i, k, j)]) -
+                               (U[g.u(i - 1, k, j)] + uc) * (W[g.w(i - 1, k, j + 1)] + W[g.w(i - 1, k, j)])) / dz);
+    }
+    if (i >= 1 && j >= 1) {
+        const double vc = V[g.v(i, k, j)];
+        as_global(f.G)[g.G(i, k, j)] =
+            vc + dt * ((V[g.v(i, k, j + 1)] - 2 * vc + V[g.v(i, k, j - 1)]) / Re / dx2 +
+                       (V[g.v(i, k + 1, j)] - 2 * vc + V[g.v(i, k - 1, j)]) / Re / dy2 +
+                       (V[g.v(i + 1, k, j)] - 2 * vc + V[g.v(i - 1, k, j)]) / Re / dz2 -
+                       (sq(0.5 * (vc + V[g.v(i, k + 1, j)])) - sq(0.5 * (V[g.v(i, k - 1, j)] + vc))) / dy -
+                       0.25 * ((U[g.u(i, k, j)] + U[g.u(i, k + 1, j)]) * (V[g.v(i, k, j + 1)] + vc) -
+                               (U[g.u(i, k, j - 1)] + U[g.u(i, k + 1, j - 1)]) * (vc + V[g.v(i, k, j - 1)])) / dx -
+                       0.25 * ((W[g.w(i, k, j)] + W[g.w(i, k + 1, j)]) * (vc + V[g.v(i + 1, k, j)]) -
+                               (W[g.w(i - 1, k, j)] + W[g.w(i - 1, k + 1, j)]) * (V[g.v(i - 1, k, j)] + vc)) / dz);
+    }
+    if (k >= 1 && j >= 1) {
+        const double wc = W[g.w(i, k, j)];
+        as_global(f.H)[g.H(i, k, j)] =
+            wc + dt * ((W[g.w(i, k, j + 1)] - 2 * wc + W[g.w(i, k, j - 1)]) / Re / dx2 +
+                       (W[g.w(i, k + 1, j)] - 2 * wc + W[g.w(i, k - 1, j)]) / Re / dy2 +
+                       (W[g.w(i + 1, k, j)] - 2 * wc + W[g.w(i - 1, k, j)]) / Re / dz2 -
+                       (sq(0.5 * (W[g.w(i + 1, k, j)] + wc)) - sq(0.5 * (W[g.w(i - 1, k, j)] + wc))) / dz -
+                       0.25 * ((U[g.u(i + 1, k, j)] + U[g.u(i, k, j)]) * (W[g.w(i, k, j + 1)] + wc) -
+                               (U[g.u(i + 1, k, j - 1)] + U[g.u(i, k, j - 1)]) * (wc + W[g.w(i, k, j - 1)])) / dx -
+                       0.25 * ((wc + W[g.w(i, k + 1, j)]) * (V[g.v(i, k, j)] + V[g.v(i + 1, k, j)]) -
+                               (W[g.w(i, k - 1, j)] + wc) * (V[g.v(i, k - 1, j)] + V[g.v(i + 1, k - 1, j)])) / dy);
+    }
+}
+
+// Poisson right-hand side with the pressure boundary values moved over
+// (src/ns_cube.cpp:204-235); interior i, k, j in [1, n]
+__global__ void ns_rhs_kernel(Grid g, Fields f) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (j > g.nx || k > g.ny) return;
+    const auto Fa = as_global((const double*)f.F);
+    const auto Ga = as_global((const double*)f.G);
+    const auto Ha = as_global((const double*)f.H);
+    const auto P = as_global((const double*)f.p);
+    const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy, dz2 = g.dz * g.dz;
+    double r = ((Fa[g.F(i, k, j)] - Fa[g.F(i, k, j - 1)]) / g.dx + (Ga[g.G(i, k, j)] - Ga[g.G(i, k - 1, j)]) / g.dy +
+                (Ha[g.H(i, k, j)] - Ha[g.H(i - 1, k, j)]) / g.dz) /
+               g.dt;
+    if (i <= 1) r -= P[g.p(i - 1, k, j)] / dz2;
+    if (k <= 1) r -= P[g.p(i, k - 1, j)] / dy2;
+    if (j <= 1) r -= P[g.p(i, k, j - 1)] / dx2;
+    if (j >= g.nx) r -= P[g.p(i, k, j + 1)] / dx2;
+    if (k >= g.ny) r -= P[g.p(i, k + 1, j)] / dy2;
+    if (i >= g.nz) r -= P[g.p(i + 1, k, j)] / dz2;
+    as_global(f.RHS)[g.x(i, k, j)] = r;
+}
+
+// update_uvwp (src/ns_cube.cpp:238-277): interior faces from F, G, H and the
+// pressure x; p's interior = x (p = x copies the index intersection)
+__global__ void ns_update_kernel(Grid g, Fields f) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (j > g.nx || k > g.ny) return;
+    const auto X = as_global((const double*)f.x);
+    const double xc = X[g.x(i, k, j)];
+    if (j < g.nx) as_global(f.u)[g.u(i, k, j)] = f.F[g.F(i, k, j)] - g.dt / g.dx * (X[g.x(i, k, j + 1)] - xc);
+    if (k < g.ny) as_global(f.v)[g.v(i, k, j)] = f.G[g.G(i, k, j)] - g.dt / g.dy * (X[g.x(i, k + 1, j)] - xc);
+    if (i < g.nz) as_global(f.w)[g.w(i, k, j)] = f.H[g.H(i, k, j)] - g.dt / g.dz * (X[g.x(i + 1, k, j)] - xc);
+    as_global(f.p)[g.p(i, k, j)] = xc;
+}
+
+}  // namespace
+
+struct fdmgpu_ns_t {
+    int device = 0;
+    Grid g{};
+    Fields f{};
+    size_t count[9] = {};
+    fdmgpu_plan plan = nullptr;
+    hipStream_t cap = nullptr;     // capture stream
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long steps = 0;
+    bool use_graph = true;
+};
+
+namespace {
+
+#define NS_HIPCHK(x)                                                                     \
+    do {                                                                                 \
+        hipError_t _e = (x);                                                             \
+        if (_e != hipSuccess) {                                                          \
+            set_error_text(hipGetErrorString(_e));                                       \
+            return _e == hipErrorOutOfMemory ? FDMGPU_ERR_OUT_OF_MEMORY : FDMGPU_ERR_DEVICE; \
+        }                                                                                \
+    } while (0)
+
+int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s) {
+    const Grid& g = N->g;
+    const dim3 b2(64, 4);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(ns_lid_kernel, dim3((g.nz + 3 + 63) / 64, g.ny + 2), dim3(64), 0, s, g, N->f.u);
+    NS_HIPCHK(hipGetLastError());
+    const int fa = std::max(g.nx, g.ny) + 2, fb = std::max(g.nz, g.ny) + 2;
+    hipLaunchKernelGGL(ns_ghost_kernel, dim3((fa + 63) / 64, fb, 3), dim3(64), 0, s, g, N->f.u, N->f.v, N->f.w);
+    NS_HIPCHK(hipGetLastError());
+    const int pa = std::max(g.nx, g.ny), pb = std::max(g.nz, g.ny);
+    hipLaunchKernelGGL(ns_pface_kernel, dim3((pa + 63) / 64, pb, 3), dim3(64), 0, s, g, N->f.u, N->f.v, N->f.w,
+                       N->f.p);
+    NS_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(ns_fgh_kernel, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1), b2, 0, s, g, N->f);
+    NS_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(ns_rhs_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
+    NS_HIPCHK(hipGetLastError());
+    int rc = plan_solve(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(ns_update_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
+    NS_HIPCHK(hipGetLastError());
+    return FDMGPU_OK;
+}
+
+void ns_free(fdmgpu_ns_t* N) {
+    if (!N) return;
+    (void)hipSetDevice(N->device);
+    if (N->cap) (void)hipStreamSynchronize(N->cap);
+    if (N->exec) (void)hipGraphExecDestroy(N->exec);
+    if (N->graph) (void)hipGraphDestroy(N->graph);
+    double* a[] = {N->f.u, N->f.v, N->f.w, N->f.p, N->f.x, N->f.F, N->f.G, N->f.H, N->f.RHS};
+    for (double* p : a) (void)hipFree(p);
+    if (N->plan) fdmgpu_destroy(N->plan);
+    if (N->ev_in) (void)hipEventDestroy(N->ev_in);
+    if (N->ev_out) (void)hipEventDestroy(N->ev_out);
+    if (N->cap) (void)hipStreamDestroy(N->cap);
+    delete N;
+}
+
+double** ns_field(fdmgpu_ns_t* N, int which) {
+    double** a[] = {&N->f.u, &N->f.v, &N->f.w, &N->f.p, &N->f.x, &N->f.F, &N->f.G, &N->f.H, &N->f.RHS};
+    return (which >= 0 && which < 9) ? a[which] : nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdmgpu_ns_create(fdmgpu_ns* ns, int solver_precision, int nx, int ny, int nz, double x1, double x2, double y1,
+                     double y2, double z1, double z2, double u0, double Re, double dt) {
+    if (!ns || (solver_precision != FDMGPU_F64 && solver_precision != FDMGPU_F32_REFINE64)) return FDMGPU_ERR_INVALID_ARG;
+    *ns = nullptr;
+    if (nx < 2 || ny < 2 || nz < 2) return FDMGPU_ERR_INVALID_SIZE;
+    if (nz > nx) return FDMGPU_ERR_UNSUPPORTED;   // the lid loop's x bound is nz+1 (src/ns_cube.cpp:68)
+    if (!(Re > 0) || !(dt > 0) || !(x2 > x1) || !(y2 > y1) || !(z2 > z1)) return FDMGPU_ERR_INVALID_ARG;
+    fdmgpu_ns_t* N = new fdmgpu_ns_t();
+    if (hipGetDevice(&N->device) != hipSuccess) {
+        delete N;
+        return FDMGPU_ERR_DEVICE;
+    }
+    Grid& g = N->g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.dx = (x2 - x1) / nx; g.dy = (y2 - y1) / ny; g.dz = (z2 - z1) / nz;
+    g.U0 = u0; g.Re = Re; g.dt = dt;
+    const size_t X = nx, Y = ny, Z = nz;
+    const size_t cnt[9] = {(Z + 2) * (Y + 2) * (X + 3), (Z + 2) * (Y + 3) * (X + 2), (Z + 3) * (Y + 2) * (X + 2),
+                           (Z + 2) * (Y + 2) * (X + 2), Z * Y * X, Z * Y * (X + 1), Z * (Y + 1) * X,
+                           (Z + 1) * Y * X, Z * Y * X};
+    int rc = FDMGPU_OK;
+    for (int i = 0; i < 9 && !rc; i++) {
+        N->count[i] = cnt[i];
+        double** p = ns_field(N, i);
+        if (hipMalloc(p, sizeof(double) * cnt[i]) != hipSuccess ||
+            hipMemset(*p, 0, sizeof(double) * cnt[i]) != hipSuccess)   // fluid at rest (zero-initialised tensors)
+            rc = FDMGPU_ERR_OUT_OF_MEMORY;
+    }
+    // the solver of NSCube::lapl_solver (src/ns_cube.h:77)
+    if (!rc) rc = fdmgpu_plan_cube(&N->plan, solver_precision, 0, g.dx, g.dy, g.dz, x2 - x1 + g.dx, y2 - y1 + g.dy,
+                                   z2 - z1 + g.dz, nx, ny, nz);
+    if (!rc && (hipStreamCreateWithFlags(&N->cap, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&N->ev_in, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&N->ev_out, hipEventDisableTiming) != hipSuccess))
+        rc = FDMGPU_ERR_DEVICE;
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = FDMGPU_ERR_DEVICE;
+    if (rc) {
+        ns_free(N);
+        return rc;
+    }
+    *ns = N;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_step(fdmgpu_ns N, int count, void* stream) {
+    if (!N || count < 0) return FDMGPU_ERR_INVALID_ARG;
+    NS_HIPCHK(hipSetDevice(N->device));
+    hipStream_t user = (hipStream_t)stream;
+    hipStream_t s = N->cap;
+    NS_HIPCHK(hipEventRecord(N->ev_in, user));
+    NS_HIPCHK(hipStreamWaitEvent(s, N->ev_in, 0));
+    for (int c = 0; c < count; c++) {
+        if (!N->use_graph || N->steps < 2) {   // eager for the first two steps (allocations, warm-up)
+            int rc = ns_enqueue_step(N, s);
+            if (rc) return rc;
+        } else {
+            if (!N->exec) {
+                NS_HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+                int rc = ns_enqueue_step(N, s);
+                hipGraph_t graph = nullptr;
+                hipError_t e = hipStreamEndCapture(s, &graph);
+                if (!rc && e != hipSuccess) rc = FDMGPU_ERR_DEVICE;
+                if (rc) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc;
+                }
+                N->graph = graph;
+                NS_HIPCHK(hipGraphInstantiate(&N->exec, N->graph, nullptr, nullptr, 0));
+            }
+            NS_HIPCHK(hipGraphLaunch(N->exec, s));
+        }
+        N->steps++;
+    }
+    if (N->steps > 0) {   // fdmgpu_get_stats on the solver plan reports the last step's pressure solve
+        N->plan->last_stream = s;
+        N->plan->last_count = 1;
+    }
+    NS_HIPCHK(hipEventRecord(N->ev_out, s));
+    NS_HIPCHK(hipStreamWaitEvent(user, N->ev_out, 0));
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_get(fdmgpu_ns N, int which, double* host, unsigned long long count) {
+    if (!N || !host) return FDMGPU_ERR_INVALID_ARG;
+    double** p = ns_field(N, which);
+    if (!p || count != N->count[which]) return FDMGPU_ERR_INVALID_ARG;
+    NS_HIPCHK(hipSetDevice(N->device));
+    NS_HIPCHK(hipStreamSynchronize(N->cap));
+    NS_HIPCHK(hipMemcpy(host, *p, sizeof(double) * count, hipMemcpyDeviceToHost));
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_set(fdmgpu_ns N, int which, const double* host, unsigned long long count) {
+    if (!N || !host) return FDMGPU_ERR_INVALID_ARG;
+    double** p = ns_field(N, which);
+    if (!p || count != N->count[which]) return FDMGPU_ERR_INVALID_ARG;
+    NS_HIPCHK(hipSetDevice(N->device));
+    NS_HIPCHK(hipStreamSynchronize(N->cap));
+    NS_HIPCHK(hipMemcpy(*p, host, sizeof(double) * count, hipMemcpyHostToDevice));
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_info(fdmgpu_ns N, int which, unsigned long long* count, void** device_ptr) {
+    if (!N) return FDMGPU_ERR_INVALID_ARG;
+    double** p = ns_field(N, which);
+    if (!p) return FDMGPU_ERR_INVALID_ARG;
+    if (count) *count = N->count[which];
+    if (device_ptr) *device_ptr = *p;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_solver(fdmgpu_ns N, fdmgpu_plan* plan) {
+    if (!N || !plan) return FDMGPU_ERR_INVALID_ARG;
+    *plan = N->plan;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_destroy(fdmgpu_ns N) {
+    if (!N) return FDMGPU_ERR_INVALID_ARG;
+    ns_free(N);
+    return FDMGPU_OK;
+}
+
+}  // extern "C"

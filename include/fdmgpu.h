@@ -216,6 +216,43 @@ int fdmgpu_server_destroy(fdmgpu_server server);
  * (read back by later processes through FDMGPU_TUNE_CACHE). */
 int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char* cache_path);
 
+/* ---- NSCube: the caller of the solve --------------------------------------
+ * Replaces NSCube<double,check>::NSCube(const Config&) (src/ns_cube.h:46-78)
+ * with the configuration values as arguments: domain [x1,x2]x[y1,y2]x[z1,z2],
+ * lid velocity u0, Re, dt, sizes; the pressure Poisson solve (the member
+ * LaplCube lapl_solver, src/ns_cube.h:77) runs at solver_precision
+ * (FDMGPU_F64 or FDMGPU_F32_REFINE64).  Fields start at rest.  nz > nx is
+ * FDMGPU_ERR_UNSUPPORTED (the reference's lid loop would index u out of
+ * range, src/ns_cube.cpp:68). */
+typedef struct fdmgpu_ns_t* fdmgpu_ns;
+int fdmgpu_ns_create(fdmgpu_ns* ns, int solver_precision, int nx, int ny, int nz, double x1, double x2, double y1,
+                     double y2, double z1, double z2, double u0, double Re, double dt);
+/* Replaces NSCube::step() (src/ns_cube.cpp:27-62), `count` times, ordered on
+ * `stream` (returns after enqueueing); steps from the third on are replayed
+ * from a hipGraph of one step. */
+int fdmgpu_ns_step(fdmgpu_ns ns, int count, void* stream);
+/* Fields with the reference's index ranges (src/ns_cube.h:66-75), x fastest:
+ * U [0,nz+1]x[0,ny+1]x[-1,nx+1], V [0,nz+1]x[-1,ny+1]x[0,nx+1],
+ * W [-1,nz+1]x[0,ny+1]x[0,nx+1], P [0,n+1]^3, X and RHS [1,n]^3,
+ * F [1,nz]x[1,ny]x[0,nx], G [1,nz]x[0,ny]x[1,nx], H [0,nz]x[1,ny]x[1,nx]. */
+#define FDMGPU_NS_U 0
+#define FDMGPU_NS_V 1
+#define FDMGPU_NS_W 2
+#define FDMGPU_NS_P 3
+#define FDMGPU_NS_X 4
+#define FDMGPU_NS_F 5
+#define FDMGPU_NS_G 6
+#define FDMGPU_NS_H 7
+#define FDMGPU_NS_RHS 8
+/* Copy a field to / from host memory (synchronises); count = its element count. */
+int fdmgpu_ns_get(fdmgpu_ns ns, int which, double* host, unsigned long long count);
+int fdmgpu_ns_set(fdmgpu_ns ns, int which, const double* host, unsigned long long count);
+/* Element count and device address of a field (device-side post-processing). */
+int fdmgpu_ns_info(fdmgpu_ns ns, int which, unsigned long long* count, void** device_ptr);
+/* The pressure solver plan (statistics, timing, refinement settings). */
+int fdmgpu_ns_solver(fdmgpu_ns ns, fdmgpu_plan* plan);
+int fdmgpu_ns_destroy(fdmgpu_ns ns);
+
 /* Replaces ~LaplCube / ~LaplRect. */
 int fdmgpu_destroy(fdmgpu_plan plan);
 

@@ -73,6 +73,15 @@ def lib():
         L.orc_rel_residual.argtypes = [_dp, _dp, _i, _i, _i, _d, _d, _d]
         L.orc_rel_residual.restype = _d
         L.orc_num_threads.restype = _i
+        L.orc_ns_create.argtypes = [_i, _i, _i] + [_d] * 9
+        L.orc_ns_create.restype = C.c_void_p
+        L.orc_ns_step.argtypes = [C.c_void_p, _i]
+        L.orc_ns_step.restype = _i
+        L.orc_ns_get.argtypes = [C.c_void_p, _i, C.c_void_p]
+        L.orc_ns_get.restype = C.c_long
+        L.orc_ns_set.argtypes = [C.c_void_p, _i, _dp]
+        L.orc_ns_set.restype = C.c_long
+        L.orc_ns_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -282,3 +291,50 @@ def manufactured_cube(n: int, seed: int, x1=-np.pi, x2=np.pi):
     u = field(seed, (n, n, n))
     f = apply_lapl(u, h, h, h)
     return u, f, h, l
+
+
+# ---------------------------------------------------------------------------
+# NSCube<double> (oracle restatement of src/ns_cube.cpp; see fdm_oracle.cpp)
+# ---------------------------------------------------------------------------
+NS_FIELDS = ("u", "v", "w", "p", "x", "F", "G", "H", "RHS")
+
+
+def ns_shapes(nx, ny, nz):
+    """Array shapes (z, y, x) with the reference's index ranges (src/ns_cube.h:66-75)."""
+    return {"u": (nz + 2, ny + 2, nx + 3), "v": (nz + 2, ny + 3, nx + 2), "w": (nz + 3, ny + 2, nx + 2),
+            "p": (nz + 2, ny + 2, nx + 2), "x": (nz, ny, nx), "F": (nz, ny, nx + 1), "G": (nz, ny + 1, nx),
+            "H": (nz + 1, ny, nx), "RHS": (nz, ny, nx)}
+
+
+class NsCube:
+    """CPU oracle of NSCube<double,check>: NSCube(Config) parameters, step()."""
+
+    def __init__(self, nx, ny, nz, x1=-np.pi, x2=np.pi, y1=-np.pi, y2=np.pi, z1=-np.pi, z2=np.pi, u0=1.0,
+                 Re=1.0, dt=0.001):
+        self.h = lib().orc_ns_create(nx, ny, nz, x1, x2, y1, y2, z1, z2, u0, Re, dt)
+        if not self.h:
+            raise ValueError("unsupported NSCube size (needs n >= 2 and nz <= nx)")
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.dx, self.dy, self.dz = (x2 - x1) / nx, (y2 - y1) / ny, (z2 - z1) / nz
+        self.dt, self.Re, self.u0 = dt, Re, u0
+        self.shapes = ns_shapes(nx, ny, nz)
+
+    def step(self, count=1):
+        rc = lib().orc_ns_step(self.h, count)
+        if rc:
+            raise RuntimeError(f"oracle NSCube step failed ({rc})")
+
+    def get(self, name):
+        out = np.empty(self.shapes[name], np.float64)
+        lib().orc_ns_get(self.h, NS_FIELDS.index(name), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def set(self, name, a):
+        a = _c(a, np.float64)
+        assert a.shape == self.shapes[name]
+        lib().orc_ns_set(self.h, NS_FIELDS.index(name), a)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_ns_destroy(self.h)
+            self.h = None

@@ -559,6 +559,161 @@ void ns_poisson_rhs(T* RHS, const T* F, const T* G, const T* H, const T* p, int 
 }
 
 // ---------------------------------------------------------------------------
+// NSCube<double> time step (src/ns_cube.h:12-78, src/ns_cube.cpp:27-277):
+// lid-driven cavity, staggered MAC grid, explicit advection + diffusion
+// (F, G, H), pressure Poisson projection through LaplCube, velocity update.
+// Arrays (x fastest), with the reference's index ranges:
+//   u  i in [0,nz+1], k in [0,ny+1], j in [-1,nx+1]   (src/ns_cube.h:66)
+//   v  i in [0,nz+1], k in [-1,ny+1], j in [0,nx+1]   (:67)
+//   w  i in [-1,nz+1], k in [0,ny+1], j in [0,nx+1]   (:68)
+//   p  i,k,j in [0,n+1]; x, RHS in [1,n]; F j in [0,nx]; G k in [0,ny];
+//   H i in [0,nz] (:69-75)
+// Quirk kept: the lid loop's x bound is nz+1 (src/ns_cube.cpp:68); a cube
+// with nz > nx would index u out of range (the reference's checked tensors
+// abort), so such sizes are rejected.
+// ---------------------------------------------------------------------------
+struct NsState {
+    int nx, ny, nz;
+    double dx, dy, dz, U0, Re, dt, lx, ly, lz;
+    std::vector<double> u, v, w, p, x, F, G, H, RHS;
+    long steps = 0;
+    double& U(int i, int k, int j) { return u[((size_t)i * (ny + 2) + k) * (nx + 3) + (j + 1)]; }
+    double& V(int i, int k, int j) { return v[((size_t)i * (ny + 3) + (k + 1)) * (nx + 2) + j]; }
+    double& W(int i, int k, int j) { return w[((size_t)(i + 1) * (ny + 2) + k) * (nx + 2) + j]; }
+    double& P(int i, int k, int j) { return p[((size_t)i * (ny + 2) + k) * (nx + 2) + j]; }
+    double& X(int i, int k, int j) { return x[((size_t)(i - 1) * ny + (k - 1)) * nx + (j - 1)]; }
+    double& Fv(int i, int k, int j) { return F[((size_t)(i - 1) * ny + (k - 1)) * (nx + 1) + j]; }
+    double& Gv(int i, int k, int j) { return G[((size_t)(i - 1) * (ny + 1) + k) * nx + (j - 1)]; }
+    double& Hv(int i, int k, int j) { return H[((size_t)i * ny + (k - 1)) * nx + (j - 1)]; }
+};
+
+static inline double sqd(double a) { return a * a; }
+
+// init_bound (src/ns_cube.cpp:64-127)
+void ns_init_bound(NsState& S) {
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    for (int k = 0; k <= ny + 1; k++)
+        for (int j = -1; j <= nz + 1; j++) S.U(nz + 1, k, j) = 2 * S.U0 - S.U(nz, k, j);
+    for (int i = 0; i <= nz + 1; i++)
+        for (int k = 0; k <= ny + 1; k++) {
+            S.U(i, k, -1) = S.U(i, k, 1);
+            S.U(i, k, nx + 1) = S.U(i, k, nx - 1);
+        }
+    for (int i = 0; i <= nz + 1; i++)
+        for (int j = 0; j <= nx + 1; j++) {
+            S.V(i, -1, j) = S.V(i, 1, j);
+            S.V(i, ny + 1, j) = S.V(i, ny - 1, j);
+        }
+    for (int k = 0; k <= ny + 1; k++)
+        for (int j = 0; j <= nx + 1; j++) {
+            S.W(-1, k, j) = S.W(1, k, j);
+            S.W(nz + 1, k, j) = S.W(nz - 1, k, j);
+        }
+    const double Re = S.Re;
+    for (int i = 1; i <= nz; i++)
+        for (int k = 1; k <= ny; k++) {
+            S.P(i, k, 0) = S.P(i, k, 1) - (S.U(i, k, 1) - 2 * S.U(i, k, 0) + S.U(i, k, -1)) / Re / S.dx;
+            S.P(i, k, nx + 1) =
+                S.P(i, k, nx) - (S.U(i, k, nx + 1) - 2 * S.U(i, k, nx) + S.U(i, k, nx - 1)) / Re / S.dx;
+        }
+    for (int i = 1; i <= nz; i++)
+        for (int j = 1; j <= nx; j++) {
+            S.P(i, 0, j) = S.P(i, 1, j) - (S.V(i, 1, j) - 2 * S.V(i, 0, j) + S.V(i, -1, j)) / Re / S.dy;
+            S.P(i, ny + 1, j) =
+                S.P(i, ny, j) - (S.V(i, ny + 1, j) - 2 * S.V(i, ny, j) + S.V(i, ny - 1, j)) / Re / S.dy;
+        }
+    for (int k = 1; k <= ny; k++)
+        for (int j = 1; j <= nx; j++) {
+            S.P(0, k, j) = S.P(1, k, j) - (S.W(1, k, j) - 2 * S.W(0, k, j) + S.W(-1, k, j)) / Re / S.dz;
+            S.P(nz + 1, k, j) =
+                S.P(nz, k, j) - (S.W(nz + 1, k, j) - 2 * S.W(nz, k, j) + S.W(nz - 1, k, j)) / Re / S.dz;
+        }
+}
+
+// FGH (src/ns_cube.cpp:131-201)
+void ns_fgh(NsState& S) {
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const double Re = S.Re, dt = S.dt, dx = S.dx, dy = S.dy, dz = S.dz;
+    const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= nz; i++)
+        for (int k = 1; k <= ny; k++)
+            for (int j = 0; j <= nx; j++) {
+                const double uc = S.U(i, k, j);
+                S.Fv(i, k, j) =
+                    uc + dt * ((S.U(i, k, j + 1) - 2 * uc + S.U(i, k, j - 1)) / Re / dx2 +
+                               (S.U(i, k + 1, j) - 2 * uc + S.U(i, k - 1, j)) / Re / dy2 +
+                               (S.U(i + 1, k, j) - 2 * uc + S.U(i - 1, k, j)) / Re / dz2 -
+                               (sqd(0.5 * (uc + S.U(i, k, j + 1))) - sqd(0.5 * (S.U(i, k, j - 1) + uc))) / dx -
+                               0.25 * ((uc + S.U(i, k + 1, j)) * (S.V(i, k, j + 1) + S.V(i, k, j)) -
+                                       (S.U(i, k - 1, j) + uc) * (S.V(i, k - 1, j + 1) + S.V(i, k - 1, j))) / dy -
+                               0.25 * ((uc + S.U(i + 1, k, j)) * (S.W(i, k, j + 1) + S.W(i, k, j)) -
+                                       (S.U(i - 1, k, j) + uc) * (S.W(i - 1, k, j + 1) + S.W(i - 1, k, j))) / dz);
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= nz; i++)
+        for (int k = 0; k <= ny; k++)
+            for (int j = 1; j <= nx; j++) {
+                const double vc = S.V(i, k, j);
+                S.Gv(i, k, j) =
+                    vc + dt * ((S.V(i, k, j + 1) - 2 * vc + S.V(i, k, j - 1)) / Re / dx2 +
+                               (S.V(i, k + 1, j) - 2 * vc + S.V(i, k - 1, j)) / Re / dy2 +
+                               (S.V(i + 1, k, j) - 2 * vc + S.V(i - 1, k, j)) / Re / dz2 -
+                               (sqd(0.5 * (vc + S.V(i, k + 1, j))) - sqd(0.5 * (S.V(i, k - 1, j) + vc))) / dy -
+                               0.25 * ((S.U(i, k, j) + S.U(i, k + 1, j)) * (S.V(i, k, j + 1) + vc) -
+                                       (S.U(i, k, j - 1) + S.U(i, k + 1, j - 1)) * (vc + S.V(i, k, j - 1))) / dx -
+                               0.25 * ((S.W(i, k, j) + S.W(i, k + 1, j)) * (vc + S.V(i + 1, k, j)) -
+                                       (S.W(i - 1, k, j) + S.W(i - 1, k + 1, j)) * (S.V(i - 1, k, j) + vc)) / dz);
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i <= nz; i++)
+        for (int k = 1; k <= ny; k++)
+            for (int j = 1; j <= nx; j++) {
+                const double wc = S.W(i, k, j);
+                S.Hv(i, k, j) =
+                    wc + dt * ((S.W(i, k, j + 1) - 2 * wc + S.W(i, k, j - 1)) / Re / dx2 +
+                               (S.W(i, k + 1, j) - 2 * wc + S.W(i, k - 1, j)) / Re / dy2 +
+                               (S.W(i + 1, k, j) - 2 * wc + S.W(i - 1, k, j)) / Re / dz2 -
+                               (sqd(0.5 * (S.W(i + 1, k, j) + wc)) - sqd(0.5 * (S.W(i - 1, k, j) + wc))) / dz -
+                               0.25 * ((S.U(i + 1, k, j) + S.U(i, k, j)) * (S.W(i, k, j + 1) + wc) -
+                                       (S.U(i + 1, k, j - 1) + S.U(i, k, j - 1)) * (wc + S.W(i, k, j - 1))) / dx -
+                               0.25 * ((wc + S.W(i, k + 1, j)) * (S.V(i, k, j) + S.V(i + 1, k, j)) -
+                                       (S.W(i, k - 1, j) + wc) * (S.V(i, k - 1, j) + S.V(i + 1, k - 1, j))) / dy);
+            }
+}
+
+template <typename T>
+int lapl_cube_solve(T* ans, const T* rhs, int periodic, double dx, double dy, double dz,
+                    double lx, double ly, double lz, int nx, int ny, int nz);
+
+// poisson + update_uvwp (src/ns_cube.cpp:204-277); p = x copies the index
+// intersection, i.e. p's interior (src/tensor.h:104-111, 282-285).
+int ns_poisson_update(NsState& S) {
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    ns_poisson_rhs<double>(S.RHS.data(), S.F.data(), S.G.data(), S.H.data(), S.p.data(), nx, ny, nz, S.dx, S.dy,
+                           S.dz, S.dt);
+    int rc = lapl_cube_solve<double>(S.x.data(), S.RHS.data(), 0, S.dx, S.dy, S.dz, S.lx, S.ly, S.lz, nx, ny, nz);
+    if (rc) return rc;
+    const double dt = S.dt;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= nz; i++)
+        for (int k = 1; k <= ny; k++)
+            for (int j = 1; j < nx; j++) S.U(i, k, j) = S.Fv(i, k, j) - dt / S.dx * (S.X(i, k, j + 1) - S.X(i, k, j));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= nz; i++)
+        for (int k = 1; k < ny; k++)
+            for (int j = 1; j <= nx; j++) S.V(i, k, j) = S.Gv(i, k, j) - dt / S.dy * (S.X(i, k + 1, j) - S.X(i, k, j));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < nz; i++)
+        for (int k = 1; k <= ny; k++)
+            for (int j = 1; j <= nx; j++) S.W(i, k, j) = S.Hv(i, k, j) - dt / S.dz * (S.X(i + 1, k, j) - S.X(i, k, j));
+    for (int i = 1; i <= nz; i++)
+        for (int k = 1; k <= ny; k++)
+            for (int j = 1; j <= nx; j++) S.P(i, k, j) = S.X(i, k, j);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Synthetic inputs (workload definition, not a reference function):
 // counter-based splitmix64, value(seed, idx) = 2*(x >> 11)*2^-53 - 1.
 // ---------------------------------------------------------------------------
@@ -660,6 +815,60 @@ void orc_ns_poisson_rhs_d(double* RHS, const double* F, const double* G, const d
                           int nx, int ny, int nz, double dx, double dy, double dz, double dt) {
     ns_poisson_rhs<double>(RHS, F, G, H, p, nx, ny, nz, dx, dy, dz, dt);
 }
+
+// ----- NSCube (constructor arguments as NSCube(const Config&), src/ns_cube.h:46-78)
+void* orc_ns_create(int nx, int ny, int nz, double x1, double x2, double y1, double y2, double z1, double z2,
+                    double U0, double Re, double dt) {
+    if (nx < 2 || ny < 2 || nz < 2 || nz > nx) return nullptr;
+    NsState* S = new NsState();
+    S->nx = nx; S->ny = ny; S->nz = nz;
+    S->dx = (x2 - x1) / nx; S->dy = (y2 - y1) / ny; S->dz = (z2 - z1) / nz;
+    S->lx = x2 - x1 + S->dx; S->ly = y2 - y1 + S->dy; S->lz = z2 - z1 + S->dz;
+    S->U0 = U0; S->Re = Re; S->dt = dt;
+    S->u.assign((size_t)(nz + 2) * (ny + 2) * (nx + 3), 0.0);
+    S->v.assign((size_t)(nz + 2) * (ny + 3) * (nx + 2), 0.0);
+    S->w.assign((size_t)(nz + 3) * (ny + 2) * (nx + 2), 0.0);
+    S->p.assign((size_t)(nz + 2) * (ny + 2) * (nx + 2), 0.0);
+    S->x.assign((size_t)nz * ny * nx, 0.0);
+    S->RHS.assign((size_t)nz * ny * nx, 0.0);
+    S->F.assign((size_t)nz * ny * (nx + 1), 0.0);
+    S->G.assign((size_t)nz * (ny + 1) * nx, 0.0);
+    S->H.assign((size_t)(nz + 1) * ny * nx, 0.0);
+    return S;
+}
+
+// NSCube::step() (src/ns_cube.cpp:27-62) `count` times; returns 0 or the solver's error.
+int orc_ns_step(void* h, int count) {
+    NsState* S = (NsState*)h;
+    for (int c = 0; c < count; c++) {
+        ns_init_bound(*S);
+        ns_fgh(*S);
+        int rc = ns_poisson_update(*S);
+        if (rc) return rc;
+        S->steps++;
+    }
+    return 0;
+}
+
+// Copies one array out: 0 u, 1 v, 2 w, 3 p, 4 x, 5 F, 6 G, 7 H, 8 RHS; returns its length.
+long orc_ns_get(void* h, int which, double* out) {
+    NsState* S = (NsState*)h;
+    std::vector<double>* a[] = {&S->u, &S->v, &S->w, &S->p, &S->x, &S->F, &S->G, &S->H, &S->RHS};
+    if (which < 0 || which > 8) return -1;
+    if (out) std::copy(a[which]->begin(), a[which]->end(), out);
+    return (long)a[which]->size();
+}
+
+// Overwrites one array (tests start from a given state).
+long orc_ns_set(void* h, int which, const double* in) {
+    NsState* S = (NsState*)h;
+    std::vector<double>* a[] = {&S->u, &S->v, &S->w, &S->p, &S->x, &S->F, &S->G, &S->H, &S->RHS};
+    if (which < 0 || which > 8) return -1;
+    std::copy(in, in + a[which]->size(), a[which]->begin());
+    return (long)a[which]->size();
+}
+
+void orc_ns_destroy(void* h) { delete (NsState*)h; }
 
 // ----- workload helpers (inputs and residual norms) ------------------------
 // u ~ U[-1,1) on the n points of the interior, seeded.
