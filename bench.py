@@ -145,19 +145,23 @@ def algorithmic_bytes(kernel: str, mode: str, iters: int, n: int, batch: int) ->
     Per point: FP64 plain: xy_fwd 8+8, zcr 8+8, xy_inv 8+8.
     FP32+refine with k inner solves (cubes whose planes fit LDS):
       solve 1:      xy_fwd f64->f32 8+4, zcr 4+4, xy_inv f32->f64 4+8;
-      solve i >= 2: xy_fwd_resid = FP64 residual fused into the forward DST
-                    (read f, u; write r32) 8+8+4, zcr 4+4, xy_inv with the
-                    update (xy_inv_acc: read c, u; write u) 4+8+8,
-                    norm-only residual certification (read f, u) 8+8.
-    Total 32 + 48 (k-1) bytes/point (96 for k = 2, the 48 k n model).
+      solve i >= 2: the rhs r32 = f - A u of the iterate, either
+                    residual_rhs (read f, u; write r32) 8+8+4 then xy_fwd_r32
+                    (read r32, write) 4+4 [default], or xy_fwd_resid, the
+                    residual fused into the forward DST, 8+8+4;
+                    zcr 4+4; xy_inv_acc (read c, u; write u) 4+8+8;
+      end:          residual, the norm-only certification (read f, u) 8+8.
+    Total 32 + 56 (k-1) + 16 bytes/point (104 for k = 2) on the default
+    path, 32 + 48 (k-1) + 16 (96) fused.
     """
     pts = float(n) ** 3 * batch
     if mode == "f64":
         per = {"xy_fwd": 16, "zcr": 16, "xy_inv": 16}.get(kernel, 0)
         return per * pts
     first = {"xy_fwd": 12, "zcr": 8, "xy_inv": 12}.get(kernel, 0)
-    later = {"xy_fwd_resid": 20, "zcr": 8, "xy_inv_acc": 20, "residual": 16}.get(kernel, 0)
-    return (first + later * max(0, iters - 1)) * pts
+    later = {"residual_rhs": 20, "xy_fwd_r32": 8, "xy_fwd_resid": 20, "zcr": 8, "xy_inv_acc": 20}.get(kernel, 0)
+    last = {"residual": 16}.get(kernel, 0)
+    return (first + later * max(0, iters - 1) + last) * pts
 
 
 def measured_traffic(kernel: str, n: int, batch: int):
@@ -578,8 +582,8 @@ def main():
     ms_total, launches = tm[dom]
     alg = algorithmic_bytes(dom, "refine", res_r["iterations"], n, B) * args.steps
     achieved = alg / (ms_total / 1e3) / 1e9
-    # whole-solve byte model 48 k n (k inner solves)
-    model_bytes = 48.0 * res_r["iterations"] * n ** 3 * solves
+    # whole-solve algorithmic bytes: the per-class model summed over the classes that ran
+    model_bytes = sum(algorithmic_bytes(k, "refine", res_r["iterations"], n, B) for k in tm) * args.steps
     err_u = float(np.linalg.norm(res_d["u0"] - us[0]) / np.linalg.norm(us[0]))
     err_r = float(np.linalg.norm(res_r["u0"] - res_d["u0"]) / np.linalg.norm(res_d["u0"]))
 

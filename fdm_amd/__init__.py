@@ -36,14 +36,14 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_rect_set_scales", "fd
            "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_group_create",
            "fdmgpu_group_solve", "fdmgpu_group_result", "fdmgpu_group_destroy", "fdmgpu_server_create",
            "fdmgpu_server_submit", "fdmgpu_server_set_refine", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
-           "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_ns_create", "fdmgpu_ns_step", "fdmgpu_ns_get", "fdmgpu_ns_set", "fdmgpu_ns_info",
+           "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_tune_set", "fdmgpu_ns_create", "fdmgpu_ns_step", "fdmgpu_ns_get", "fdmgpu_ns_set", "fdmgpu_ns_info",
            "fdmgpu_ns_solver", "fdmgpu_ns_destroy", "fdmgpu_destroy", "fdmgpu_strerror",
            "fdmgpu_version")
 
 # kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
 K_XY_FWD, K_XY_INV, K_X_TILE, K_Y_TILE, K_ZCR, K_ROWCR, K_RESIDUAL, K_OTHER = range(8)
 KERNEL_NAMES = ("xy_fwd", "xy_inv", "x_tile", "y_tile", "zcr", "rowcr", "residual", "other", "xy_fwd_resid",
-                "xy_inv_acc")
+                "xy_inv_acc", "residual_rhs", "xy_fwd_r32")
 
 
 class FdmGpuError(RuntimeError):
@@ -131,6 +131,7 @@ def _lib_handle():
         L.fdmgpu_server_forget.argtypes = [vp, ull]
         L.fdmgpu_server_destroy.argtypes = [vp]
         L.fdmgpu_tune.argtypes = [i, C.POINTER(Problem), i, C.c_char_p]
+        L.fdmgpu_tune_set.argtypes = [C.POINTER(Problem), C.POINTER(Variant)]
         L.fdmgpu_ns_create.argtypes = [C.POINTER(vp), i, i, i, i] + [d] * 9
         L.fdmgpu_ns_step.argtypes = [vp, i, vp]
         L.fdmgpu_ns_get.argtypes = [vp, i, vp, ull]

@@ -143,6 +143,129 @@ __global__ __launch_bounds__(256) void zcr_kernel(const BatchPtr<Ts> data, const
 }
 
 // ---------------------------------------------------------------------------
+// Strip layout (Dirichlet): a wavefront holds one 32-row chunk of 64
+// consecutive systems, lane = system, so every load and store instruction of
+// a wave covers 64 consecutive modes of one z-plane (256 B in FP32, 512 B in
+// FP64) instead of the lane layout's G short runs.  The G = 2^Q / 32 chunk
+// waves of a strip exchange their separator contributions through LDS (one
+// barrier).  S strips per block.  The arithmetic is the lane layout's: the
+// same levels in the same order, so results are identical.
+// ---------------------------------------------------------------------------
+template <int Q>
+struct ZcrStripCfg {
+    static constexpr int NQ = 1 << Q;
+    static constexpr int C = NQ >= 32 ? 32 : NQ;   // rows per lane (incl. separator slot)
+    static constexpr int G = NQ / C;               // chunk waves per strip
+    static constexpr int LOGG = ilog2(G);
+    static constexpr int LOC = G == 1 ? Q - 1 : Q - LOGG;
+};
+
+template <typename Ts, typename T, int Q, int S>
+__global__ __launch_bounds__(64 * ZcrStripCfg<Q>::G * S) void zcr_strip_kernel(
+    const BatchPtr<Ts> data, const long plane, const int nx, const Ts* __restrict__ lmx,
+    const Ts* __restrict__ lmy, const T b, const T d0, const T gamma, const int* __restrict__ done) {
+    using Cf = ZcrStripCfg<Q>;
+    constexpr int NQ = Cf::NQ, C = Cf::C, G = Cf::G, LOGG = Cf::LOGG, LOC = Cf::LOC;
+    constexpr int SYS = 64 * S;   // systems per block
+    __shared__ T xch[G > 1 ? 2 * G * SYS : 1];
+    if (done && done[blockIdx.y]) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = w % G, strip = w / G;
+    const int sys = strip * 64 + lane;
+    const long m = (long)blockIdx.x * SYS + sys;
+    const bool valid = m < plane;
+    const long mm = valid ? m : 0;
+    const auto base = as_global(data(blockIdx.y) + mm);
+    const int ky = (int)(mm / nx), kx = (int)(mm - (long)ky * nx);
+
+    T v[C];
+#pragma unroll
+    for (int r = 0; r < C; ++r) {
+        const int row = c * C + r;
+        v[r] = (valid && row < NQ - 1) ? (T)base[(long)row * plane] * gamma : T(0);
+    }
+    CrLevels<T, Q> L;
+    cr_levels<T, Q>(L, b, d0 - (T)lmx[kx] - (T)lmy[ky]);
+
+    // forward reduction inside the chunk
+    T toRight = T(0), toLeft = T(0);
+    static_for<1, LOC + 1>([&](auto ll) {
+        constexpr int l = decltype(ll)::value;
+        constexpr int st = 1 << l, h = st >> 1;
+        const T a = L.alpha[l];
+#pragma unroll
+        for (int r = st - 1; r < C - 1; r += st) v[r] += a * (v[r - h] + v[r + h]);
+        if constexpr (G > 1) {
+            toRight += a * v[C - 1 - h];
+            toLeft += a * v[h - 1];
+        }
+    });
+
+    T xleft = T(0);
+    if constexpr (G == 1) {
+        v[C / 2 - 1] *= L.inv_d[Q - 1];
+        v[C - 1] = T(0);
+    } else {
+        // separator c (row (c+1)*C-1): this chunk's part and the next chunk's toLeft
+        T* tl = xch;
+        T* sv = xch + G * SYS;
+        tl[c * SYS + sys] = toLeft;
+        sv[c * SYS + sys] = (c < G - 1) ? v[C - 1] + toRight : T(0);
+        __syncthreads();
+        T sp[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) sp[i] = (i < G - 1) ? sv[i * SYS + sys] + tl[(i + 1) * SYS + sys] : T(0);
+        static_for<1, LOGG>([&](auto ll) {
+            constexpr int lp = decltype(ll)::value;
+            constexpr int st = 1 << lp, h = st >> 1;
+            const T a = L.alpha[LOC + lp];
+#pragma unroll
+            for (int i = st - 1; i < G - 1; i += st) sp[i] += a * (sp[i - h] + sp[i + h]);
+        });
+        sp[G / 2 - 1] *= L.inv_d[Q - 1];
+        static_for<1, LOGG>([&](auto ll) {
+            constexpr int lp = LOGG - decltype(ll)::value;   // LOGG-1 .. 1
+            constexpr int st = 1 << lp, h = st >> 1;
+            const T cc = L.c[LOC + lp - 1], id = L.inv_d[LOC + lp - 1];
+#pragma unroll
+            for (int i = h - 1; i < G - 1; i += st) {
+                const T left = i - h >= 0 ? sp[i - h] : T(0);
+                const T right = i + h < G - 1 ? sp[i + h] : T(0);
+                sp[i] = (sp[i] - cc * (left + right)) * id;
+            }
+        });
+        T xl = T(0), xr = T(0);
+#pragma unroll
+        for (int i = 0; i < G - 1; ++i) {
+            if (i == c - 1) xl = sp[i];
+            if (i == c) xr = sp[i];
+        }
+        xleft = xl;
+        v[C - 1] = xr;
+    }
+
+    // back substitution inside the chunk
+    static_for<0, LOC>([&](auto ll) {
+        constexpr int l = LOC - decltype(ll)::value;   // LOC .. 1
+        constexpr int st = 1 << l, h = st >> 1;
+        const T cc = L.c[l - 1], id = L.inv_d[l - 1];
+#pragma unroll
+        for (int r = h - 1; r < C - 1; r += st) {
+            const T left = r - h >= 0 ? v[r - h] : xleft;
+            v[r] = (v[r] - cc * (left + v[r + h])) * id;
+        }
+    });
+
+    if (valid) {
+#pragma unroll
+        for (int r = 0; r < C; ++r) {
+            const int row = c * C + r;
+            if (row < NQ - 1) base[(long)row * plane] = (Ts)v[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Periodic z (LaplCube<..., periodic^3>): after the half-complex real DFT in
 // x and y, mode (kx,ky) obeys the circulant tridiagonal system
 //   b u_{i-1} + d u_i + b u_{i+1} = g_i,  i = 0..n-1 (indices mod n), n = 2^Q,
@@ -374,9 +497,13 @@ __global__ __launch_bounds__(256) void rowcr_kernel(const BatchPtr<Ts> data, con
 }
 
 // Lane multipliers a z system of 2^q rows supports: 1 always, 2 when every
-// lane still holds at least 2 rows.
+// lane still holds at least 2 rows; the strip layouts (3: 64 systems per
+// block, 4: 128) from 32 rows up (Dirichlet only), 4 up to 128 rows (512
+// threads, registers stay within budget in FP64).
 bool zcr_gmul_ok(int q, int gm) {
     if (gm == 1) return true;
+    if (gm == 3) return q >= 5 && q <= 9;
+    if (gm == 4) return q >= 5 && q <= 7;
     if (gm != 2 || q < 2) return false;
     const int NQ = 1 << q;
     const int G = (NQ > 64 ? NQ / 64 : 1) * gm;
@@ -395,12 +522,29 @@ static hipError_t zcr_k(const ZcrArgs& a) {
     return hipGetLastError();
 }
 
+template <typename Ts, typename T, int Q, int S>
+static hipError_t zcr_strip_k(const ZcrArgs& a) {
+    constexpr int SYS = 64 * S;
+    const unsigned blocks = (unsigned)((a.plane + SYS - 1) / SYS);
+    const BatchPtr<Ts> data{(Ts*)a.data, a.data_tab, a.bs};
+    hipLaunchKernelGGL((zcr_strip_kernel<Ts, T, Q, S>), dim3(blocks, a.batch), dim3(64 * ZcrStripCfg<Q>::G * S), 0,
+                       a.stream, data, a.plane, a.nx, (const Ts*)a.lmx, (const Ts*)a.lmy, (T)a.b, (T)a.d0,
+                       (T)a.gamma, a.done);
+    return hipGetLastError();
+}
+
 template <typename Ts, typename T, int Q>
 static hipError_t zcr_g(const ZcrArgs& a) {
-    // default: 2 lanes per 64 rows (32 rows per lane) from 64 rows up: fewer
-    // registers per lane, twice the occupancy (measured faster at 128 rows)
-    const int gm = a.gmul ? a.gmul : (Q >= 6 ? 2 : 1);
-    if (!zcr_gmul_ok(Q, gm)) return hipErrorInvalidValue;
+    // default: Dirichlet systems of 32 to 512 rows in the strip layout (64
+    // consecutive modes per wave; measured fastest at 128 rows), circulant ones
+    // with 2 lanes per 64 rows from 64 rows up (fewer registers per lane,
+    // twice the occupancy of 1)
+    const int gm = a.gmul ? a.gmul : (!a.periodic && zcr_gmul_ok(Q, 3)) ? 3 : (Q >= 6 ? 2 : 1);
+    if (!zcr_gmul_ok(Q, gm) || (gm >= 3 && a.periodic)) return hipErrorInvalidValue;
+    if (gm == 3) return zcr_strip_k<Ts, T, Q, 1>(a);
+    if constexpr (ZcrStripCfg<Q>::G <= 4) {
+        if (gm == 4) return zcr_strip_k<Ts, T, Q, 2>(a);
+    }
     if (gm == 1) return zcr_k<Ts, T, Q, 1>(a);
     if constexpr (Q >= 2) {
         if (gm == 2) return zcr_k<Ts, T, Q, 2>(a);

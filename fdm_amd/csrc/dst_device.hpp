@@ -178,7 +178,9 @@ struct DstLane {
 // writes 16 eight-byte banks per quarter-wavefront.  A job's N1 lanes touch N1
 // slots that are distinct mod N1, so a row pitch congruent to N1 mod 32 makes
 // row-pencil (x) accesses conflict-free.  Column-pencil (y) accesses are then
-// spread by XOR-ing the column index with a function of the row, f(row).
+// spread by shifting each row by f(row) < N1 slots (the pitch leaves room).
+// The shift is additive rather than an XOR so that, for a fixed row, slot
+// offsets stay compile-time constants a ds instruction can carry.
 template <int N1>
 struct RowSwizzle {
     static constexpr int QJ = 32 / N1;                            // jobs per half-wavefront
@@ -188,14 +190,39 @@ struct RowSwizzle {
     __device__ static constexpr int f(int row) { return ((row >> LS) & FM) << LS; }
 };
 
-// Plane of rows with pitch P; element (row, col) at row*P + (col ^ f(row)).
+// Hides a value's derivation from the optimizer (no instruction is emitted).
+__device__ __forceinline__ int opaque(int x) {
+    asm("" : "+v"(x));
+    return x;
+}
+
+// Plane of rows with pitch P; element (row, col) at row*P + col + f(row).
 // Pencils are rows (x pass) or columns (y pass).
 template <int P, int N1, bool ALONG_X>
 struct PlaneLayout {
     using Sw = RowSwizzle<N1>;
-    __device__ static constexpr int at(int row, int col) { return row * P + (col ^ Sw::f(row)); }
+    __device__ static constexpr int at(int row, int col) { return row * P + col + Sw::f(row); }
     __device__ static constexpr int addr(int pencil, int slot) {
         return ALONG_X ? at(pencil, slot) : at(slot, pencil);
+    }
+    // addr(pencil + DP, lv + cp) for a lane-dependent pencil and lv and
+    // compile-time DP and cp, cp a multiple of N1.  f() has period N1 (it reads
+    // only bits below log2(N1)), so f(lv + cp) = f(lv), and likewise
+    // f(pencil + DP) = f(pencil) when N1 divides DP: the constant parts then
+    // become an offset the ds instruction carries, and the lane part is
+    // computed once per distinct (pencil, lv).  The lane part goes through
+    // opaque() so the compiler cannot fold the constants back into the
+    // row multiply.
+    template <int DP>
+    __device__ static int addr_split(int pencil, int lv, int cp) {
+        if constexpr (ALONG_X) {
+            if constexpr (DP % N1 == 0 || Sw::FM == 0)
+                return opaque(pencil * P + Sw::f(pencil) + lv) + (DP * P + cp);
+            else
+                return opaque((pencil + DP) * P + Sw::f(pencil + DP) + lv) + cp;
+        } else {
+            return opaque(lv * P + Sw::f(lv) + pencil) + (DP + cp * P);
+        }
     }
 };
 
@@ -267,22 +294,25 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
         const int w = base + tid;
         if (w < WORK) {
             const int g = w / N1;
-            const int gb = g + JOBS;
-#define PA(j) lds[Lay::addr(g, (j))]
-#define PB(j) lds[Lay::addr(gb, (j))]
+// slot = lv + cp: lv lane-dependent, cp a compile-time multiple of N1 (see
+// PlaneLayout::addr_split); pencil gb = g + JOBS sits at a constant offset
+#define PA(lv, cp) lds[Lay::template addr_split<0>(g, (lv), (cp))]
+#define PB(lv, cp) lds[Lay::template addr_split<JOBS>(g, (lv), (cp))]
 
             // ---- step 1: pre-processing + N2-point DFTs over j2 (j = t + N1*j2)
             T yr[N2], yi[N2];
+            const int tn0 = (N - t) & (N - 1);   // slot N - j for j = t (0 for j = 0)
 #pragma unroll
             for (int j2 = 0; j2 < N2; ++j2) {
                 const int j = t + N1 * j2;
                 if constexpr (KIND == kRdftFwd) {
-                    yr[j2] = PA(j);
-                    yi[j2] = PB(j);
+                    yr[j2] = PA(t, N1 * j2);
+                    yi[j2] = PB(t, N1 * j2);
                 } else {
-                    const int jn = j2 == 0 ? ((N - t) & (N - 1)) : N - j;   // slot 0 for j = 0
-                    const T a0 = PA(j), a1 = PA(jn);
-                    const T b0 = PB(j), b1 = PB(jn);
+                    // N - j = (N1 - t) + (N - N1 - N1*j2) for j2 > 0
+                    const int jl = j2 == 0 ? tn0 : N1 - t, jc = j2 == 0 ? 0 : N - N1 - N1 * j2;
+                    const T a0 = PA(t, N1 * j2), a1 = PA(jl, jc);
+                    const T b0 = PB(t, N1 * j2), b1 = PB(jl, jc);
                     if constexpr (KIND == kDst) {
                         const T s = L.sn[j2];
                         yr[j2] = s * (a0 + a1) + T(0.5) * (a0 - a1);
@@ -309,10 +339,10 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
             }
             wave_lds_sync();
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) {
-                const int pos = S::posY(t, k2);
-                PA(pos) = yr[k2];
-                PB(pos) = yi[k2];
+            for (int k2 = 0; k2 < N2; ++k2) {   // posY(t, k2) = N1*k2 + (t ^ (k2 mod N1))
+                const int lv = t ^ (k2 & (N1 - 1));
+                PA(lv, N1 * k2) = yr[k2];
+                PB(lv, N1 * k2) = yi[k2];
             }
             wave_lds_sync();
 
@@ -321,13 +351,12 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
             T zr[N2], zi[N2];
 #pragma unroll
             for (int r = 0; r < RR; ++r) {
-                const int k2 = t + N1 * r;
                 T vr[N1], vi[N1];
 #pragma unroll
-                for (int j1 = 0; j1 < N1; ++j1) {
-                    const int pos = S::posY(j1, k2);
-                    vr[j1] = PA(pos);
-                    vi[j1] = PB(pos);
+                for (int j1 = 0; j1 < N1; ++j1) {   // posY(j1, k2) = N1*t + (j1 ^ t) + N1*N1*r
+                    const int lv = N1 * t + (j1 ^ t);
+                    vr[j1] = PA(lv, N1 * N1 * r);
+                    vi[j1] = PB(lv, N1 * N1 * r);
                 }
                 dft<N1>(vr, vi);
 #pragma unroll
@@ -341,13 +370,16 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
             for (int r = 0; r < RR; ++r) {
 #pragma unroll
                 for (int k1 = 0; k1 < N1; ++k1) {
-                    const int pos = S::posZ(t + N1 * r + N2 * k1);
+                    // posZ permutes aligned blocks of KC <= N1 slots, so
+                    // posZ(t + cz) - cz < N1 for the multiple of N1 cz
+                    const int cz = N1 * r + N2 * k1;
+                    const int lv = S::posZ(t + cz) - cz;
                     if constexpr (KIND == kRdftInv) {   // x^a = Re Y, x^b = -Im Y
-                        PA(pos) = zr[r * N1 + k1] * scale;
-                        PB(pos) = -zi[r * N1 + k1] * scale;
+                        PA(lv, cz) = zr[r * N1 + k1] * scale;
+                        PB(lv, cz) = -zi[r * N1 + k1] * scale;
                     } else {
-                        PA(pos) = zr[r * N1 + k1];
-                        PB(pos) = zi[r * N1 + k1];
+                        PA(lv, cz) = zr[r * N1 + k1];
+                        PB(lv, cz) = zi[r * N1 + k1];
                     }
                 }
             }
@@ -361,12 +393,12 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                     const int k = t * KC + i;
                     const bool k0 = k == 0;
                     const int pk = S::posZ(k), pn = S::posZ(k0 ? N / 2 : N - k);
-                    const T zkr = PA(pk), zki = PB(pk);
-                    const T znr = PA(pn), zni = PB(pn);
-                    PA(pk) = (k0 ? zkr : T(0.5) * (zkr + znr)) * scale;
-                    PB(pk) = (k0 ? zki : T(0.5) * (zki + zni)) * scale;
-                    PA(pn) = (k0 ? znr : T(0.5) * (zni - zki)) * scale;
-                    PB(pn) = (k0 ? zni : T(0.5) * (zkr - znr)) * scale;
+                    const T zkr = PA(pk, 0), zki = PB(pk, 0);
+                    const T znr = PA(pn, 0), zni = PB(pn, 0);
+                    PA(pk, 0) = (k0 ? zkr : T(0.5) * (zkr + znr)) * scale;
+                    PB(pk, 0) = (k0 ? zki : T(0.5) * (zki + zni)) * scale;
+                    PA(pn, 0) = (k0 ? znr : T(0.5) * (zni - zki)) * scale;
+                    PB(pn, 0) = (k0 ? zni : T(0.5) * (zkr - znr)) * scale;
                 }
                 wave_lds_sync();
             } else if constexpr (KIND == kDst) {
@@ -377,8 +409,8 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                 for (int i = 0; i < KC; ++i) {
                     const int k = t * KC + i;
                     const int pk = S::posZ(k), pn = S::posZ((N - k) & (N - 1));
-                    const T zkr = PA(pk), zki = PB(pk);
-                    const T znr = PA(pn), zni = PB(pn);
+                    const T zkr = PA(pk, 0), zki = PB(pk, 0);
+                    const T znr = PA(pn, 0), zni = PB(pn, 0);
                     ae[i] = T(0.5) * (zni - zki);
                     be[i] = T(0.5) * (zkr - znr);
                     T dA = T(0.5) * (zkr + znr);
@@ -404,14 +436,17 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                 wave_lds_sync();
 #pragma unroll
                 for (int i = 0; i < KC; ++i) {
+                    // sigma(2k + e) = t*N2 + ((2i + e) ^ t) for k = t*KC + i (2i + e < N2, t < N1):
+                    // lane part t*N2 + (((2i + e) mod N1) ^ t), the rest a multiple of N1
                     const int k = t * KC + i;
-                    const int pe = S::sigma(2 * k), po = S::sigma(2 * k + 1);
                     const T e_a = k == 0 ? T(0) : ae[i] * scale;
                     const T e_b = k == 0 ? T(0) : be[i] * scale;
-                    PA(pe) = e_a;
-                    PB(pe) = e_b;
-                    PA(po) = ao[i] * scale;
-                    PB(po) = bo[i] * scale;
+                    const int me = 2 * i, mo = 2 * i + 1;
+                    const int le = t * N2 + ((me & (N1 - 1)) ^ t), lo = t * N2 + ((mo & (N1 - 1)) ^ t);
+                    PA(le, me & ~(N1 - 1)) = e_a;
+                    PB(le, me & ~(N1 - 1)) = e_b;
+                    PA(lo, mo & ~(N1 - 1)) = ao[i] * scale;
+                    PB(lo, mo & ~(N1 - 1)) = bo[i] * scale;
                 }
                 wave_lds_sync();
             }

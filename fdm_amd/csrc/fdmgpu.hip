@@ -132,8 +132,9 @@ size_t work_bytes_shape(int kind, int precision, int nx, int ny, int nz, int cou
     if (precision != FDMGPU_F32_REFINE64 || count <= 0) return 0;
     const long nblk = std::max<long>(residual_partials(nx, ny, nz, kind == 3), nz);
     const size_t elems = (size_t)nx * ny * nz;
-    return align256(sizeof(float) * elems * count) + align256(sizeof(double) * 2 * nblk * count) +
-           2 * align256(sizeof(int) * count) + align256(sizeof(double) * count);
+    return (kind == 3 ? 2 : 1) * align256(sizeof(float) * elems * count) +
+           align256(sizeof(double) * 2 * nblk * count) + 2 * align256(sizeof(int) * count) +
+           align256(sizeof(double) * count);
 }
 
 size_t work_bytes(const fdmgpu_plan_t* P, int count) {
@@ -147,6 +148,10 @@ Work work_carve(const fdmgpu_plan_t* P, int count, void* mem) {
     const long nblk = partial_blocks(P);
     W.r32 = (float*)c;
     c += align256(sizeof(float) * plan_elems(P) * count);
+    if (P->kind == 3) {
+        W.u32 = (float*)c;
+        c += align256(sizeof(float) * plan_elems(P) * count);
+    }
     W.partial = (double*)c;
     c += align256(sizeof(double) * 2 * nblk * count);
     W.done = (int*)c;
@@ -159,7 +164,7 @@ Work work_carve(const fdmgpu_plan_t* P, int count, void* mem) {
 }
 
 void work_free(Work& W) {
-    hipFree(W.r32);
+    hipFree(W.r32);   // u32 lives in the same allocation
     hipFree(W.partial);
     hipFree(W.done);
     hipFree(W.iters);
@@ -182,7 +187,9 @@ int work_reserve(const fdmgpu_plan_t* P, Work& W, int count) {
     W.fcopy_bytes = fcb;
     const long elems = plan_elems(P);
     const long nblk = partial_blocks(P);
-    HIPCHK(hipMalloc(&W.r32, sizeof(float) * elems * count));
+    const size_t r32b = align256(sizeof(float) * elems * count);
+    HIPCHK(hipMalloc(&W.r32, P->kind == 3 ? 2 * r32b : r32b));
+    if (P->kind == 3) W.u32 = (float*)((char*)W.r32 + r32b);
     HIPCHK(hipMalloc(&W.partial, sizeof(double) * 2 * nblk * count));
     HIPCHK(hipMalloc(&W.done, sizeof(int) * count));
     HIPCHK(hipMalloc(&W.iters, sizeof(int) * count));
@@ -206,6 +213,9 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
     };
     std::vector<int> gms{1};
     if (P->kind == 3 && zcr_gmul_ok(P->logz, 2)) gms.push_back(2);
+    if (P->kind == 3 && !(P->periodic & FDMGPU_PERIODIC_Z))
+        for (int gm : {3, 4})
+            if (zcr_gmul_ok(P->logz, gm)) gms.push_back(gm);
     std::vector<Variant> out;
     Variant d;
     d.plane = plane_capable(P) ? 1 : 0;
@@ -289,12 +299,12 @@ using Tab = const void* const*;
 // Forward xy transforms: in (Tin) -> out (T).
 template <typename Tin, typename T>
 int stage_fwd(fdmgpu_plan_t* P, const void* in, Tab in_tab, void* out, Tab out_tab, int count, hipStream_t s,
-              const int* done) {
+              const int* done, int plane_class = FDMGPU_K_XY_FWD) {
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = P->dx * std::sqrt(2. / P->lx), sy = P->dy * std::sqrt(2. / P->ly);
     if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
         PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done, P->kfwd, in_tab, out_tab};
-        Timed tm(P, FDMGPU_K_XY_FWD, s);
+        Timed tm(P, plane_class, s);
         HIPCHK((launch_plane<Tin, T, T>(P->logx, a)));
         return FDMGPU_OK;
     }
@@ -319,18 +329,21 @@ int stage_fwd(fdmgpu_plan_t* P, const void* in, Tab in_tab, void* out, Tab out_t
     return FDMGPU_OK;
 }
 
-// Inverse xy transforms: in (T) -> out (Tout), optionally accumulating.
+// Inverse xy transforms: in (T) -> out (Tout), optionally accumulating; with
+// an FP32 `base` (plane path only) out = base + result.
 template <typename T, typename Tout>
 int stage_inv(fdmgpu_plan_t* P, void* in, Tab in_tab, void* out, Tab out_tab, int count, hipStream_t s, bool accum,
-              const int* done) {
+              const int* done, const float* base = nullptr) {
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = std::sqrt(2. / P->lx), sy = std::sqrt(2. / P->ly);
     if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
-        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv, in_tab, out_tab};
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv, in_tab, out_tab,
+                    base};
         Timed tm(P, accum ? FDMGPU_K_XY_INV_ACC : FDMGPU_K_XY_INV, s);
         HIPCHK((launch_plane<T, Tout, T>(P->logx, a)));
         return FDMGPU_OK;
     }
+    if (base) return FDMGPU_ERR_INVALID_ARG;   // only the plane path adds onto a separate base
     if (P->kind == 3) {
         AxisArgs ay{in, in, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv, in_tab,
                     in_tab, P->var.tile_y};
@@ -404,6 +417,10 @@ __global__ void add_f32_to_f64(const BatchPtr<double> u, const float* __restrict
 //   solve i >= 2: r = f - A u (FP64, fused into the forward plane DST for
 //                 cubes), ||r||/||f|| -> per-problem flag, c = solve32(r), u += c
 //   finally:      ||f - A u||/||f|| of the last iterate.
+// On the plane path the iterate after solve 1 holds FP32 values, so when a
+// second solve follows in the same call it stays in the FP32 scratch u32
+// (4 bytes per point written and read twice instead of 8); solve 2 writes
+// u = u32 + c into ans, and problems that converged on solve 1 get u = u32.
 // A problem whose residual reached rtol is flagged on the device; every later
 // launch skips it, so the host never synchronises inside a solve.
 // first > 1 continues a refinement whose iterate (in ans) after `first - 1`
@@ -455,6 +472,10 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
     ra.f_tab = rhs_tab;
     ra.u_tab = ans_tab;
     const bool fused = plane_capable(P) && P->var.plane && P->var.fuse;
+    // u1 in FP32 scratch: separate-residual plane path (Dirichlet planes), solves 1 and 2 in
+    // this call (the fused forward pass reads an FP32 iterate slower than it saves)
+    const bool u1f = plane_capable(P) && P->var.plane && !fused && P->kinv == kDst && W.u32 && first == 1 &&
+                     last >= 2;
 
     // Fused path (cubes whose planes fit LDS): the forward DST of solve i >= 2
     // computes r = f - A u itself; convergence of the iterate after solve i is
@@ -463,7 +484,7 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
     if (first > 1 && !fused) {   // the next solve's rhs: r32 = f - A u of the current iterate
         ra.it = first - 1;
         ra.r32 = W.r32;
-        Timed tm(P, FDMGPU_K_RESIDUAL, s);
+        Timed tm(P, FDMGPU_K_RESIDUAL_RHS, s);
         HIPCHK(launch_residual(ra));
     }
     for (int it = first; it <= last; ++it) {
@@ -483,12 +504,16 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
             }
             rc = 0;
         } else {
-            rc = stage_fwd<float, float>(P, W.r32, nullptr, W.r32, nullptr, count, s, W.done);
+            rc = stage_fwd<float, float>(P, W.r32, nullptr, W.r32, nullptr, count, s, W.done, FDMGPU_K_XY_FWD_R32);
         }
         if (rc) return rc;
         rc = stage_tridiag<float>(P, W.r32, nullptr, count, s, W.done);
         if (rc) return rc;
-        rc = stage_inv<float, double>(P, W.r32, nullptr, ans, ans_tab, count, s, it > 1, W.done);
+        if (u1f && it == 1)
+            rc = stage_inv<float, float>(P, W.r32, nullptr, W.u32, nullptr, count, s, false, W.done);
+        else
+            rc = stage_inv<float, double>(P, W.r32, nullptr, ans, ans_tab, count, s, it > 1, W.done,
+                                          u1f && it == 2 ? W.u32 : nullptr);
         if (rc == -1) {   // rect accumulate
             dim3 g((unsigned)((elems + 255) / 256), count);
             Timed tm(P, FDMGPU_K_OTHER, s);
@@ -501,7 +526,10 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
         if (fused && it == 1 && last > 1) continue;   // solve 2's fused pass checks it
         ra.it = it;
         ra.r32 = (fused || it == last) ? nullptr : W.r32;
-        Timed tm(P, FDMGPU_K_RESIDUAL, s);
+        ra.u = u1f && it == 1 ? (const double*)W.u32 : ans;
+        ra.u_tab = u1f && it == 1 ? nullptr : ans_tab;
+        ra.u_f32 = u1f && it == 1;
+        Timed tm(P, ra.r32 ? FDMGPU_K_RESIDUAL_RHS : FDMGPU_K_RESIDUAL, s);
         HIPCHK(launch_residual(ra));
     }
     return FDMGPU_OK;

@@ -46,6 +46,8 @@ struct PlaneArgs {
     int kind;             // kDst (0), kRdftFwd (1), kRdftInv (2); periodic kinds need Tin == Tout == T
     const void* const* in_tab;    // per-problem pointers (device) instead of in + p*bs_in, or null
     const void* const* out_tab;
+    const float* base = nullptr;  // with accum: out = base + result (base contiguous, stride bs_out);
+                                  // problems flagged done get out = base
 };
 
 struct AxisArgs {

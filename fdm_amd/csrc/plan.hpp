@@ -17,6 +17,7 @@ namespace fdmgpu {
 // every in-flight batch its own, so batches of one plan can overlap.
 struct Work {
     float* r32 = nullptr;               // [cap][elems]
+    float* u32 = nullptr;               // [cap][elems] cubes: the iterate after inner solve 1 (FP32 values)
     double* partial = nullptr;          // [cap][nblk][2]
     int* done = nullptr;                // [cap]
     int* iters = nullptr;               // [cap]
@@ -34,8 +35,10 @@ struct Variant {
     int tile_x = 0;     // x tile kernel: rows per block (16/32/64)
     int tile_y = 0;     // y tile kernel: columns per block
     int zcr_gmul = 0;   // z cyclic reduction: lanes-per-system multiplier (1/2)
-    int fuse = 1;       // FP32+refine on the plane path: 1 = residual fused into the forward
+    int fuse = 0;       // FP32+refine on the plane path: 1 = residual fused into the forward
                         // plane transform, 0 = separate residual pass writing the FP32 rhs
+                        // (the default: the tiled residual kernel streams faster than the
+                        // plane kernel computes the stencil)
 };
 
 }  // namespace fdmgpu

@@ -59,24 +59,31 @@ __device__ __forceinline__ void plane_sweep(const int tid, F&& f) {
 
 // KIND (dst_device.hpp): kDst for Dirichlet planes of (N-1)^2 points,
 // kRdftFwd / kRdftInv for periodic planes of N^2 points.
-template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND>
+// ACC: 0 out = result; 1 out += result; 2 out = base + result with an FP32
+// base (the refinement's first iterate), and out = base for problems flagged
+// done (converged on that iterate).
+template <typename Tin, typename Tout, typename T, int LOGN, int ACC, int KIND>
 __global__ __launch_bounds__(PlaneCfg<LOGN>::NT)
 __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void plane_dst_kernel(
     const BatchPtr<const Tin> in, const BatchPtr<Tout> out, const T sx, const T sy, const T* __restrict__ tables,
-    const int* __restrict__ done) {
+    const int* __restrict__ done, const float* __restrict__ base, const long base_bs) {
     using C = PlaneCfg<LOGN>;
     using S = DstShape<LOGN>;
     using IO = PencilIO<KIND, LOGN>;
     using LX = PlaneLayout<C::P, S::N1, true>;
     using LY = PlaneLayout<C::P, S::N1, false>;
-    if (done && done[blockIdx.y]) return;
     constexpr int N = C::N, n = IO::n, o = IO::off, NT = C::NT;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lds = reinterpret_cast<T*>(smem_raw);
     const int tid = threadIdx.x;
     const long plane_off = (long)blockIdx.x * n * n;
-    const auto src = as_global(in(blockIdx.y) + plane_off);
     const auto dst = as_global(out(blockIdx.y) + plane_off);
+    [[maybe_unused]] const auto bp = as_global(ACC == 2 ? base + blockIdx.y * base_bs + plane_off : base);
+    if (done && done[blockIdx.y]) {
+        if constexpr (ACC == 2) plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) { dst[y * n + x] = (Tout)bp[y * n + x]; });
+        return;
+    }
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    const auto src = as_global(in(blockIdx.y) + plane_off);
 
     DstLane<T, LOGN> L;
     L.init(tables, tid & (S::N1 - 1));
@@ -94,7 +101,8 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
     __syncthreads();
     plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) {
         const T v = lds[LX::at(IO::out(y), IO::out(x))];
-        if constexpr (ACCUM) dst[y * n + x] += (Tout)v;
+        if constexpr (ACC == 1) dst[y * n + x] += (Tout)v;
+        else if constexpr (ACC == 2) dst[y * n + x] = (Tout)bp[y * n + x] + (Tout)v;
         else dst[y * n + x] = (Tout)v;
     });
 }
@@ -202,28 +210,34 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<float>()))) vo
     plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) { dst[y * n + x] = lds[LX::at(S::sigma(y + 1), S::sigma(x + 1))]; });
 }
 
-template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND>
+template <typename Tin, typename Tout, typename T, int LOGN, int ACC, int KIND>
 static hipError_t plane_k(const PlaneArgs& a) {
     using C = PlaneCfg<LOGN>;
-    auto k = plane_dst_kernel<Tin, Tout, T, LOGN, ACCUM, KIND>;
+    auto k = plane_dst_kernel<Tin, Tout, T, LOGN, ACC, KIND>;
     const size_t lds = C::template lds_bytes<T>();
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     const BatchPtr<const Tin> in{(const Tin*)a.in, a.in_tab, a.bs_in};
     const BatchPtr<Tout> out{(Tout*)a.out, a.out_tab, a.bs_out};
     hipLaunchKernelGGL(k, dim3(a.nplanes, a.batch), dim3(C::NT), lds, a.stream, in, out, (T)a.sx, (T)a.sy,
-                       (const T*)a.tables, a.done);
+                       (const T*)a.tables, a.done, a.base, a.bs_out);
     return hipGetLastError();
 }
 
 template <typename Tin, typename Tout, typename T, int LOGN>
 static hipError_t plane_t(const PlaneArgs& a) {
-    if (a.kind == kDst)
-        return a.accum ? plane_k<Tin, Tout, T, LOGN, true, kDst>(a) : plane_k<Tin, Tout, T, LOGN, false, kDst>(a);
+    if (a.base && !a.accum) return hipErrorInvalidValue;
+    if (a.kind == kDst) {
+        if constexpr (std::is_same_v<Tout, double> && !std::is_same_v<Tin, double>) {
+            if (a.base) return plane_k<Tin, Tout, T, LOGN, 2, kDst>(a);
+        }
+        if (a.base) return hipErrorInvalidValue;
+        return a.accum ? plane_k<Tin, Tout, T, LOGN, 1, kDst>(a) : plane_k<Tin, Tout, T, LOGN, 0, kDst>(a);
+    }
     if constexpr (std::is_same_v<Tin, T> && std::is_same_v<Tout, T>) {
         if (a.accum) return hipErrorInvalidValue;
-        if (a.kind == kRdftFwd) return plane_k<Tin, Tout, T, LOGN, false, kRdftFwd>(a);
-        if (a.kind == kRdftInv) return plane_k<Tin, Tout, T, LOGN, false, kRdftInv>(a);
+        if (a.kind == kRdftFwd) return plane_k<Tin, Tout, T, LOGN, 0, kRdftFwd>(a);
+        if (a.kind == kRdftInv) return plane_k<Tin, Tout, T, LOGN, 0, kRdftInv>(a);
     }
     return hipErrorInvalidValue;
 }

@@ -97,56 +97,154 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
     }
 }
 
-// 3-D residual marching along z: a block of 64 x 4 threads covers
-// x in [x0, x0+64), y in [y0, y0+4) and walks z; each thread keeps u at
-// z-1, z, z+1 of its column in registers, so every u value is read from HBM
-// once and the x / y neighbours come from L1/L2 (the neighbouring threads'
-// loads of the same plane).  partial[p][blk] as residual_kernel, blk =
-// blockIdx.y * gridDim.x + blockIdx.x.
-constexpr int kZX = 64, kZY = 4;
+// 3-D residual: a block of TX x TY threads owns the tile x in [x0, x0+TX),
+// y in [y0, y0+TY) of one z-chunk and marches through it.  The u plane of the
+// current z sits in LDS (two buffers, one barrier per plane) with a one-point
+// halo, so the x / y neighbours are LDS reads; u at z-1, z, z+1 stay in
+// registers.  Loads run ahead of use: while plane z is computed, u at z+2
+// and f and the halo values at z+1 are in flight.  Each u value is read from
+// HBM once per chunk plus the chunk's two boundary planes and the tile's halo
+// rows.  Tiles are as wide as the rows where possible (no x halo).
+// Blocks go round-robin over the 8 XCDs; the logical block index (problem,
+// chunk, tile) is dealt out in contiguous slabs per XCD, so neighbouring
+// tiles, whose halo rows overlap, share an L2.
+// Norm partials: one per (z-granule of kZGran planes, tile, wave), written by
+// the wave that covers it: partial[p][(g * tiles + tile) * NW + w].  The
+// chunk length (a whole number of granules, chosen per launch to fill the
+// GPU) changes neither the layout nor the summation order.
+constexpr int kZGran = 8;
+constexpr int kXcdCount = 8;
 
-template <bool WRITE>
-__global__ __launch_bounds__(kZX* kZY) void residual_z_kernel(
-    const BatchPtr<const double> f, const BatchPtr<const double> u, float* __restrict__ r32,
+struct ResTile {
+    int tx, ty;
+};
+
+static ResTile res_tile(int nx) {
+    if (nx <= 32) return {32, 32};
+    if (nx <= 64) return {64, 16};
+    return {128, 8};
+}
+
+template <int TX, int TY, bool WRITE, typename Tu>
+__global__ __launch_bounds__(TX* TY) void residual_tile_kernel(
+    const BatchPtr<const double> f, const BatchPtr<const Tu> u, float* __restrict__ r32,
     double* __restrict__ partial, const int nx, const int ny, const int nz, const double ix, const double iy,
-    const double iz, const long bs, const int* __restrict__ done) {
-    __shared__ double sh[2 * kResBlock];
-    const int p = blockIdx.z;
+    const double iz, const long bs, const int batch, const int chunk_gran, const int* __restrict__ done) {
+    constexpr int NT = TX * TY, NW = NT / 64;
+    __shared__ double t[2][TY + 2][TX + 2];
+    const int gtx = (nx + TX - 1) / TX, gty = (ny + TY - 1) / TY;
+    const int ngran = (nz + kZGran - 1) / kZGran, zc = (ngran + chunk_gran - 1) / chunk_gran;
+    const int tiles = gtx * gty, per_problem = tiles * zc;
+    const long total = (long)per_problem * batch;   // gridDim.x = per * kXcdCount >= total
+    const long lin = blockIdx.x;
+    const long per = (total + kXcdCount - 1) / kXcdCount;
+    const long logical = (lin % kXcdCount) * per + lin / kXcdCount;
+    if (logical >= total) return;
+    const int p = (int)(logical / per_problem);
+    const int blk = (int)(logical - (long)p * per_problem);   // chunk * tiles + tile
     if (done && done[p]) return;
+    const int chunk = blk / tiles, tile = blk - chunk * tiles;
+    const int x0 = (tile % gtx) * TX, y0 = (tile / gtx) * TY;
+    const int z0 = chunk * chunk_gran * kZGran, z1 = min(nz, z0 + chunk_gran * kZGran);
     const auto fp = as_global(f(p));
     const auto up = as_global(u(p));
-    const int x = blockIdx.x * kZX + (threadIdx.x & (kZX - 1));
-    const int y = blockIdx.y * kZY + (threadIdx.x / kZX);
-    const bool valid = x < nx && y < ny;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int x = x0 + tx, y = y0 + ty;
     const long pl = (long)nx * ny;
-    const long c0 = (long)y * nx + x;
-    const bool hx0 = x > 0, hx1 = x < nx - 1, hy0 = y > 0, hy1 = y < ny - 1;
+    const bool valid = x < nx && y < ny;
+    const long c0 = valid ? (long)y * nx + x : 0;
+    // halo loads: row y0-1 by ty == 0, row y0+TY by ty == TY-1, columns x0-1 / x0+TX by tx == 0 / TX-1
+    const bool hN = ty == 0 && y > 0 && x < nx;
+    const bool hS = ty == TY - 1 && y + 1 < ny && x < nx;
+    const int xh = tx == 0 ? x - 1 : x + 1;
+    const bool hW = (tx == 0 || tx == TX - 1) && y < ny && xh >= 0 && xh < nx;
+    const long cN = hN ? (long)(y - 1) * nx + x : 0, cS = hS ? (long)(y + 1) * nx + x : 0;
+    const long cW = hW ? (long)y * nx + xh : 0;
+    const int wcol = tx == 0 ? 0 : TX + 1;
+    for (int i = threadIdx.x; i < 2 * (TY + 2) * (TX + 2); i += NT) (&t[0][0][0])[i] = 0.0;
+
+    long o = (long)z0 * pl;
+    double um = (valid && z0 > 0) ? (double)up[o - pl + c0] : 0.0;
+    double uc = valid ? (double)up[o + c0] : 0.0;
+    double un = (valid && z0 + 1 < nz) ? (double)up[o + pl + c0] : 0.0;
+    double fv = valid ? (double)fp[o + c0] : 0.0;
+    double hn = hN ? (double)up[o + cN] : 0.0, hs = hS ? (double)up[o + cS] : 0.0;
+    double hw = hW ? (double)up[o + cW] : 0.0;
     double rr = 0.0, ff = 0.0;
-    if (valid) {
-        double um = 0.0, uc = up[c0];
-#pragma unroll 4
-        for (int z = 0; z < nz; ++z) {
-            const long c = z * pl + c0;
-            const double un = z + 1 < nz ? up[c + pl] : 0.0;
-            const double xm = hx0 ? up[c - 1] : 0.0, xp = hx1 ? up[c + 1] : 0.0;
-            const double ym = hy0 ? up[c - nx] : 0.0, yp = hy1 ? up[c + nx] : 0.0;
-            const double au = (xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy + (um - 2.0 * uc + un) * iz;
-            const double fv = fp[c];
-            const double r = fv - au;
-            if constexpr (WRITE) r32[p * bs + c] = (float)r;
+    __syncthreads();
+    for (int z = z0; z < z1; ++z, o += pl) {
+        auto& b = t[z & 1];
+        b[ty + 1][tx + 1] = uc;
+        if (hN) b[0][tx + 1] = hn;
+        if (hS) b[TY + 1][tx + 1] = hs;
+        if (hW) b[ty + 1][wcol] = hw;
+        // in flight while plane z is computed: u at z+2 (plane z+1's upper
+        // neighbour), f and the halos at z+1
+        const bool more = z + 1 < z1;
+        const double unn = (valid && more && z + 2 < nz) ? (double)up[o + 2 * pl + c0] : 0.0;
+        const double fn = (valid && more) ? (double)fp[o + pl + c0] : 0.0;
+        const double hn2 = (hN && more) ? (double)up[o + pl + cN] : 0.0;
+        const double hs2 = (hS && more) ? (double)up[o + pl + cS] : 0.0;
+        const double hw2 = (hW && more) ? (double)up[o + pl + cW] : 0.0;
+        __syncthreads();   // b complete; the other buffer's readers (plane z-1) are past this point
+        const double au = (b[ty + 1][tx] - 2.0 * uc + b[ty + 1][tx + 2]) * ix +
+                          (b[ty][tx + 1] - 2.0 * uc + b[ty + 2][tx + 1]) * iy + (um - 2.0 * uc + un) * iz;
+        const double r = fv - au;
+        if (valid) {
             rr += r * r;
             ff += fv * fv;
-            um = uc;
-            uc = un;
+            if constexpr (WRITE) r32[p * bs + o + c0] = (float)r;
         }
+        if ((z & (kZGran - 1)) == kZGran - 1 || z + 1 == nz) {   // granule done: the wave's partial
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) {
+                rr += __shfl_xor(rr, d, 64);
+                ff += __shfl_xor(ff, d, 64);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                const long slot = ((long)p * ngran * tiles + (long)(z / kZGran) * tiles + tile) * NW + (threadIdx.x >> 6);
+                partial[slot * 2 + 0] = rr;
+                partial[slot * 2 + 1] = ff;
+            }
+            rr = 0.0;
+            ff = 0.0;
+        }
+        um = uc;
+        uc = un;
+        un = unn;
+        fv = fn;
+        hn = hn2;
+        hs = hs2;
+        hw = hw2;
     }
-    block_sum2<double>(rr, ff, sh);
-    if (threadIdx.x == 0) {
-        const int nblk = gridDim.x * gridDim.y;
-        const int b = blockIdx.y * gridDim.x + blockIdx.x;
-        partial[((long)p * nblk + b) * 2 + 0] = rr;
-        partial[((long)p * nblk + b) * 2 + 1] = ff;
-    }
+}
+
+// z-chunk length in granules: whole columns unless the batch is too small to
+// give every CU a couple of blocks
+static int res_chunk_gran(int nx, int ny, int nz, int batch, int tiles) {
+    const int ngran = (nz + kZGran - 1) / kZGran;
+    const long want = 2 * 256;
+    const long have = (long)tiles * batch;
+    if (have >= want) return ngran;
+    const int zc = (int)std::min<long>(ngran, (want + have - 1) / have);
+    return (ngran + zc - 1) / zc;
+}
+
+template <int TX, int TY, typename Tu>
+static void launch_residual_tile(const ResidualArgs& a, const BatchPtr<const double>& f,
+                                 const BatchPtr<const Tu>& u) {
+    const int tiles = ((a.nx + TX - 1) / TX) * ((a.ny + TY - 1) / TY);
+    const int cg = res_chunk_gran(a.nx, a.ny, a.nz, a.batch, tiles);
+    const int ngran = (a.nz + kZGran - 1) / kZGran, zc = (ngran + cg - 1) / cg;
+    // the kernel re-deals the blocks (problem, chunk, tile) to XCD slabs
+    const long total = (long)tiles * zc * a.batch;
+    const dim3 grid((unsigned)((total + kXcdCount - 1) / kXcdCount * kXcdCount));
+    if (a.r32)
+        hipLaunchKernelGGL((residual_tile_kernel<TX, TY, true, Tu>), grid, dim3(TX * TY), 0, a.stream, f, u, a.r32,
+                           a.partial, a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.batch, cg, a.done);
+    else
+        hipLaunchKernelGGL((residual_tile_kernel<TX, TY, false, Tu>), grid, dim3(TX * TY), 0, a.stream, f, u, a.r32,
+                           a.partial, a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.batch, cg, a.done);
 }
 
 // One block per problem: fixed-order sum of the partials, then the flag.
@@ -186,13 +284,19 @@ hipError_t launch_residual(const ResidualArgs& a) {
     dim3 grid(nblk, a.batch);
     const BatchPtr<const double> f{a.f, a.f_tab, a.bs}, u{a.u, a.u_tab, a.bs};
     if (a.nz > 1 || a.three) {
-        dim3 zg((a.nx + kZX - 1) / kZX, (a.ny + kZY - 1) / kZY, a.batch);
-        if (a.r32)
-            hipLaunchKernelGGL((residual_z_kernel<true>), zg, dim3(kZX * kZY), 0, a.stream, f, u, a.r32, a.partial,
-                               a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.done);
-        else
-            hipLaunchKernelGGL((residual_z_kernel<false>), zg, dim3(kZX * kZY), 0, a.stream, f, u, a.r32, a.partial,
-                               a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.done);
+        const ResTile t = res_tile(a.nx);
+        if (a.u_f32) {
+            const BatchPtr<const float> u32{(const float*)a.u, a.u_tab, a.bs};
+            if (t.tx == 32) launch_residual_tile<32, 32>(a, f, u32);
+            else if (t.tx == 64) launch_residual_tile<64, 16>(a, f, u32);
+            else launch_residual_tile<128, 8>(a, f, u32);
+        } else {
+            if (t.tx == 32) launch_residual_tile<32, 32>(a, f, u);
+            else if (t.tx == 64) launch_residual_tile<64, 16>(a, f, u);
+            else launch_residual_tile<128, 8>(a, f, u);
+        }
+    } else if (a.u_f32) {
+        return hipErrorInvalidValue;
     } else if (a.sy) {
         hipLaunchKernelGGL((residual_kernel<false, true>), grid, dim3(kResBlock), 0, a.stream, f, u, a.r32,
                            a.partial, a.nx, a.ny, 1, a.ix, a.iy, 0.0, a.sy, a.lx, a.ux, a.bs, a.done);
@@ -213,10 +317,15 @@ hipError_t launch_check(const double* partial, int nblk, double tol, int* done, 
     return hipGetLastError();
 }
 
-// partials per problem: 3-D, one per residual_z_kernel block; 2-D, chunks of
-// about 4 rows, a multiple of 8 (one slab per XCD), at most kResMaxBlocks
+// partials per problem: 3-D, one per (z-granule, tile, wave) of
+// residual_tile_kernel; 2-D, chunks of about 4 rows, a multiple of 8 (one
+// slab per XCD), at most kResMaxBlocks
 long residual_partials(int nx, int ny, int nz, bool three) {
-    if (three) return (long)((nx + kZX - 1) / kZX) * ((ny + kZY - 1) / kZY);
+    if (three) {
+        const ResTile t = res_tile(nx);
+        return (long)((nx + t.tx - 1) / t.tx) * ((ny + t.ty - 1) / t.ty) * ((nz + kZGran - 1) / kZGran) *
+               (t.tx * t.ty / 64);
+    }
     const long rows = (long)ny * nz;
     long b = (rows + 3) / 4;
     b = (b + 7) / 8 * 8;

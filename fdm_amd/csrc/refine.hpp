@@ -28,6 +28,7 @@ struct ResidualArgs {
     hipStream_t stream;
     const void* const* f_tab;   // per-problem pointers (device) or null
     const void* const* u_tab;
+    bool u_f32 = false;         // u holds FP32 values (3-D only)
 };
 
 hipError_t launch_residual(const ResidualArgs& a);

@@ -762,6 +762,23 @@ int fdmgpu_server_destroy(fdmgpu_server S) {
 }
 
 // ---- autotuner ---------------------------------------------------------------
+int fdmgpu_tune_set(const fdmgpu_problem* problem, const fdmgpu_variant* v) {
+    if (!problem || !v) return FDMGPU_ERR_INVALID_ARG;
+    int rc = check_problem(*problem);
+    if (rc) return rc;
+    fdmgpu_plan plan = nullptr;
+    if ((rc = make_plan(key_of(*problem), &plan))) return rc;
+    const Variant w{v->plane, v->tile_x, v->tile_y, v->zcr_gmul, v->fuse};
+    const bool ok = variant_valid(plan, w);
+    fdmgpu_destroy(plan);
+    if (!ok) return FDMGPU_ERR_INVALID_ARG;
+    const fdmgpu_problem& p = *problem;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_env_locked();
+    g_tune[tune_key(p.kind, p.precision, p.periodic, p.nx, p.ny, p.kind == FDMGPU_KIND_RECT ? 1 : p.nz)] = w;
+    return FDMGPU_OK;
+}
+
 int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char* cache_path) {
     if (count < 1 || !problems || reps < 1 || reps > 1000) return FDMGPU_ERR_INVALID_ARG;
     for (int i = 0; i < count; i++) {

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-from . import F32_REFINE64, FdmGpuError, Problem, Result, _check, _lib_handle, _stream_handle
+from . import F32_REFINE64, FdmGpuError, Problem, Result, Variant, _check, _lib_handle, _stream_handle
 
 KIND_RECT, KIND_CUBE = 2, 3
 GROUP_GRAPH = 1
@@ -114,3 +114,10 @@ def tune(problems, reps: int = 5, cache_path: str | None = None):
 
 
 __all__ = ["Group", "Server", "tune", "cube_problem", "rect_problem", "KIND_CUBE", "KIND_RECT", "FdmGpuError"]
+
+
+def tune_set(problem: Problem, variant) -> None:
+    """Pins `variant` (plane, tile_x, tile_y, zcr_gmul, fuse) for problems of
+    `problem`'s shape in the process-wide table (fdmgpu_tune_set)."""
+    v = Variant(*variant)
+    _check(_lib_handle().fdmgpu_tune_set(C.byref(problem), C.byref(v)), "tune_set")

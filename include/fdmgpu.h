@@ -100,10 +100,13 @@ int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_ta
 /* Kernel variants (tile shapes) of a plan, as the autotuner chooses them:
  * plane = 1 runs square planes that fit LDS through the plane kernel (0: x
  * and y tile kernels); tile_x / tile_y = pencils per block of the tile
- * kernels (16, 32, 64; 0 = default); zcr_gmul = lanes per z system
- * multiplier (1, 2; 0 = 2 from 64 rows, else 1); fuse (FP32+refine plane
+ * kernels (16, 32, 64; 0 = default); zcr_gmul = z cyclic reduction
+ * layout: 1, 2 = lanes per system multiplier inside a wavefront, 3, 4 = one
+ * wavefront per 32-row chunk of 64 consecutive modes, 64 (3) or 128 (4)
+ * modes per block (Dirichlet z, 32 to 512 rows; 4 up to 128); 0 = default
+ * (3 where valid, else 2 from 64 rows, else 1); fuse (FP32+refine plane
  * path) = FP64 residual fused into the next forward transform (1) or a
- * separate residual pass (0).  Every variant solves the same system; results
+ * separate residual pass (0, the default).  Every variant solves the same system; results
  * differ at rounding level.  No reference counterpart. */
 typedef struct fdmgpu_variant {
     int plane, tile_x, tile_y, zcr_gmul;
@@ -144,7 +147,9 @@ int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resid
 #define FDMGPU_K_OTHER 7    /* flag fills, FP32->FP64 updates, copies */
 #define FDMGPU_K_XY_FWD_RESID 8 /* FP64 residual fused into the forward plane DST (refinement solves >= 2) */
 #define FDMGPU_K_XY_INV_ACC 9   /* inverse plane DST adding the correction into the FP64 iterate */
-#define FDMGPU_NUM_KERNELS 10
+#define FDMGPU_K_RESIDUAL_RHS 10 /* FP64 residual writing the next inner solve's FP32 rhs (+ norm) */
+#define FDMGPU_K_XY_FWD_R32 11   /* forward plane DST of that FP32 rhs (refinement solves >= 2) */
+#define FDMGPU_NUM_KERNELS 12
 int fdmgpu_set_timing(fdmgpu_plan plan, int enable);
 int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launches);
 
@@ -215,6 +220,12 @@ int fdmgpu_server_destroy(fdmgpu_server server);
  * in the process-wide table, and in the JSON file `cache_path` when not NULL
  * (read back by later processes through FDMGPU_TUNE_CACHE). */
 int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char* cache_path);
+/* Pins kernel variant *v for problems of the shape of *problem in the
+ * process-wide table (as a tuner run that found it would): plans and server
+ * batches created afterwards for that shape use it.  FDMGPU_ERR_INVALID_ARG
+ * if the variant is not valid for the shape (fdmgpu_plan_variants).  No
+ * reference counterpart. */
+int fdmgpu_tune_set(const fdmgpu_problem* problem, const fdmgpu_variant* v);
 
 /* ---- NSCube: the caller of the solve --------------------------------------
  * Replaces NSCube<double,check>::NSCube(const Config&) (src/ns_cube.h:46-78)

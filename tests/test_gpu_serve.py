@@ -228,7 +228,9 @@ def test_server_admission(fdm, hip, srv):
     assert e.value.code == fdm.ERR_ADMISSION
     small.close()
     # a budget for one request's scratch at a time: three requests still complete, one by one
-    one = n ** 3 * 4 + (256 << 10)
+    resident, per_solve = fdm.LaplCube(*g, n, n, n, precision="refine").footprint(1)
+    assert per_solve >= 2 * n ** 3 * 4        # FP32 rhs/correction and first iterate
+    one = per_solve + (256 << 10)
     S = srv.Server(hbm_budget=one, max_batch=4)
     ts = [S.submit(srv.cube_problem(*g, n, n, n, o, d_f)) for o in outs]
     S.drain()
@@ -259,16 +261,32 @@ def test_tune_writes_cache_and_applies(fdm, hip, srv, tmp_path):
     assert f'"plane": {v[0]}, "tile_x": {v[1]}, "tile_y": {v[2]}, "zcr_gmul": {v[3]}' in line
 
 
-@pytest.mark.parametrize("shape", [(31, 31, 31), (63, 31, 15)])
-def test_server_continuation_matches_uninterrupted(fdm, hip, srv, shape):
+@pytest.mark.parametrize("shape,fuse", [((31, 31, 31), 0), ((31, 31, 31), 1), ((63, 31, 15), 0)])
+def test_server_continuation_matches_uninterrupted(fdm, hip, srv, shape, fuse):
     """A request that needs more inner solves than its solver's earlier requests
     continues from its iterate in a follow-up batch; the answer is bitwise the
-    uninterrupted solve's (fused plane path and the tile path)."""
+    uninterrupted solve's (plane path with the separate and the fused
+    residual, and the tile path)."""
     nx, ny, nz = shape
-    say("server continuation", shape)
+    say("server continuation", shape, fuse)
     g = geom_cube(nx, ny, nz)
     u, f = manufactured((nz, ny, nx), 41)
     s = fdm.LaplCube(*g, nx, ny, nz, precision="refine")
+    d_x, d_y = hip.DeviceArray(f.shape, np.float64), hip.DeviceArray(f.shape, np.float64)
+    shape_problem = srv.cube_problem(*g, nx, ny, nz, d_x, d_y)
+    default = s.variant
+    if nx == ny:   # pin the residual variant for this shape (plans and server batches)
+        v = next(x for x in s.variants() if x[0] == 1 and x[4] == fuse)
+        srv.tune_set(shape_problem, v)
+        s.variant = v
+    try:
+        _continuation_case(fdm, hip, srv, s, g, nx, ny, nz, f)
+    finally:
+        if nx == ny:
+            srv.tune_set(shape_problem, default)
+
+
+def _continuation_case(fdm, hip, srv, s, g, nx, ny, nz, f):
     s.set_refine(1e-30, 2)
     ans = np.zeros_like(f)
     with pytest.raises(fdm.FdmGpuError):        # 2 inner solves cannot reach 1e-30
@@ -289,3 +307,16 @@ def test_server_continuation_matches_uninterrupted(fdm, hip, srv, shape):
     assert r.status == 0 and r.iterations == 3 and r.rel_residual <= tol
     assert np.array_equal(d_b.download(), ref)
     S.close()
+
+
+def test_tune_set_rejects_invalid_variant(fdm, hip, srv):
+    """fdmgpu_tune_set accepts only variants valid for the shape."""
+    say("tune_set invalid")
+    g = geom_cube(63, 31, 15)
+    d_x, d_y = hip.DeviceArray((15, 31, 63), np.float64), hip.DeviceArray((15, 31, 63), np.float64)
+    p = srv.cube_problem(*g, 63, 31, 15, d_x, d_y)
+    with pytest.raises(fdm.FdmGpuError) as e:      # non-square planes have no plane kernel
+        srv.tune_set(p, (1, 0, 0, 0, 0))
+    assert e.value.code == fdm.ERR_INVALID_ARG
+    with pytest.raises(fdm.FdmGpuError):           # no such tile
+        srv.tune_set(p, (0, 48, 0, 0, 0))
