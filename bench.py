@@ -139,29 +139,144 @@ def unixbench_score(ts):
 # ---------------------------------------------------------------------------
 # GPU legs
 # ---------------------------------------------------------------------------
-def algorithmic_bytes(kernel: str, mode: str, iters: int, n: int, batch: int) -> float:
+def algorithmic_bytes(kernel: str, mode: str, iters: int, n: int, batch: int, fused: bool = False) -> float:
     """Algorithmic HBM bytes per solve of `batch` problems for one kernel class.
 
-    Per point: FP64 plain: xy_fwd 8+8, zcr 8+8, xy_inv 8+8.
-    FP32+refine with k inner solves (cubes whose planes fit LDS):
-      solve 1:      xy_fwd f64->f32 8+4, zcr 4+4, xy_inv f32->f64 4+8;
-      solve i >= 2: the rhs r32 = f - A u of the iterate, either
-                    residual_rhs (read f, u; write r32) 8+8+4 then xy_fwd_r32
-                    (read r32, write) 4+4 [default], or xy_fwd_resid, the
-                    residual fused into the forward DST, 8+8+4;
-                    zcr 4+4; xy_inv_acc (read c, u; write u) 4+8+8;
-      end:          residual, the norm-only certification (read f, u) 8+8.
-    Total 32 + 56 (k-1) + 16 bytes/point (104 for k = 2) on the default
-    path, 32 + 48 (k-1) + 16 (96) fused.
+    Per point: FP64 plain: xy_fwd 8+8, zcr 8+8, xy_inv 8+8 (48).
+    FP32+refine with k inner solves (cubes whose planes fit LDS), default path:
+      solve 1:  xy_fwd f64->f32 8+4, zcr 4+4, xy_inv f32->f32 4+4 (the
+                iterate stays in FP32 scratch when a solve 2 follows; 4+8
+                into the answer when k = 1);
+      solve 2:  residual_rhs (read f, FP32 u; write r32) 8+4+4, xy_fwd_r32
+                4+4, zcr 4+4, xy_inv_acc (read c, FP32 u; write u) 4+4+8;
+      solve i >= 3: the same with the FP64 iterate: 8+8+4, 4+4, 4+4, 4+8+8;
+      end:      residual, the norm-only certification (read f, u) 8+8.
+    k = 2: 28 + 48 + 16 = 92 bytes/point.  Fused variant (fused=True): solve
+    i >= 2 is xy_fwd_resid (residual inside the forward DST) 8+8+4, zcr 4+4,
+    xy_inv_acc 4+8+8, and solve 1 writes the FP64 iterate: 96 at k = 2.
     """
     pts = float(n) ** 3 * batch
     if mode == "f64":
         per = {"xy_fwd": 16, "zcr": 16, "xy_inv": 16}.get(kernel, 0)
         return per * pts
-    first = {"xy_fwd": 12, "zcr": 8, "xy_inv": 12}.get(kernel, 0)
-    later = {"residual_rhs": 20, "xy_fwd_r32": 8, "xy_fwd_resid": 20, "zcr": 8, "xy_inv_acc": 20}.get(kernel, 0)
-    last = {"residual": 16}.get(kernel, 0)
-    return (first + later * max(0, iters - 1) + last) * pts
+    k = max(1, iters)
+    if fused:
+        first = {"xy_fwd": 12, "zcr": 8, "xy_inv": 12}.get(kernel, 0)
+        later = {"xy_fwd_resid": 20, "zcr": 8, "xy_inv_acc": 20}.get(kernel, 0)
+        return (first + later * (k - 1) + {"residual": 16}.get(kernel, 0)) * pts
+    first = {"xy_fwd": 12, "zcr": 8, "xy_inv": 8 if k >= 2 else 12}.get(kernel, 0)
+    second = {"residual_rhs": 16, "xy_fwd_r32": 8, "zcr": 8, "xy_inv_acc": 16}.get(kernel, 0) if k >= 2 else 0
+    later = {"residual_rhs": 20, "xy_fwd_r32": 8, "zcr": 8, "xy_inv_acc": 20}.get(kernel, 0)
+    return (first + second + later * max(0, k - 2) + {"residual": 16}.get(kernel, 0)) * pts
+
+
+# Kernels of the 127^3 batch-32 workload as rocprofv3 names them: (key, label,
+# symbol prefix, algorithmic bytes per point per launch).  Both FP32->FP32
+# plane launches (solve 1's inverse into the FP32 iterate, solve 2's forward
+# of the residual) are one symbol with the same bytes.
+PROFILE_KERNELS = [
+    ("xy_fwd", "forward planes, f64 -> f32 (solve 1)", "plane_dst_kernel<double, float, float, 7, 0, 0>", 12),
+    ("plane_f32", "planes f32 -> f32 (solve 1 inverse, solve 2 forward)", "plane_dst_kernel<float, float, float, 7, 0, 0>", 8),
+    ("zcr", "z cyclic reduction (FP32)", "zcr_strip_kernel<float, float, 7, 1>", 8),
+    ("residual_rhs", "FP64 residual -> FP32 rhs (solve 2)", "residual_tile_kernel<128, 8, true, float>", 16),
+    ("xy_inv_acc", "inverse planes, u = u1 + c (solve 2)", "plane_dst_kernel<float, double, float, 7, 2, 0>", 16),
+    ("residual", "FP64 residual, norm only", "residual_tile_kernel<128, 8, false, double>", 16),
+    ("xy_f64", "planes FP64 (plain solve)", "plane_dst_kernel<double, double, double, 7, 0, 0>", 16),
+    ("zcr_f64", "z cyclic reduction (FP64)", "zcr_strip_kernel<double, double, 7, 1>", 16),
+]
+# timing class -> profile key, where they differ
+PROFILE_KEY_OF_CLASS = {"xy_inv": "plane_f32", "xy_fwd_r32": "plane_f32"}
+
+
+def profile_summary(kt: str, pmc: str, n: int = 127, batch: int = 32):
+    """Writes profiles/lapl_cube_127_batch32_kernels.md and
+    profiles/hbm_traffic_lapl_cube_127_batch32.json from the rocprofv3 csv
+    output of the commands listed in the markdown (kernel trace in `kt`,
+    FETCH_SIZE / WRITE_SIZE passes and their copy calibration in `pmc`)."""
+    import csv
+    import collections
+
+    pts = float(n) ** 3 * batch
+
+    def key_of(name):
+        for key, _, sym, _ in PROFILE_KERNELS:
+            if sym in name:
+                return key
+        return None
+
+    trace = list(csv.DictReader(open(os.path.join(kt, "kt_kernel_trace.csv"))))
+    stats = list(csv.DictReader(open(os.path.join(kt, "kt_kernel_stats.csv"))))
+    dur = collections.defaultdict(list)
+    for r in trace:
+        k = key_of(r["Kernel_Name"])
+        if k:
+            dur[k].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+
+    def counter(name, pre=""):
+        path = os.path.join(pmc, pre + name, f"{pre}{name}_counter_collection.csv")
+        agg = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            k = "copy" if "copyBuffer" in r["Kernel_Name"] else key_of(r["Kernel_Name"])
+            if k:
+                agg[k].append(float(r["Counter_Value"]))
+        return agg
+
+    fetch, write = counter("FETCH_SIZE"), counter("WRITE_SIZE")
+    cf, cw = counter("FETCH_SIZE", "copy_"), counter("WRITE_SIZE", "copy_")
+    copy_kib = 256 * 1024
+    kf = copy_kib / statistics.mean(cf["copy"])
+    kw = copy_kib / statistics.mean(cw["copy"])
+    out = {"n": n, "batch": batch, "fetch_scale": kf, "write_scale": kw, "kernels": {}}
+    rows = []
+    for key, label, sym, bpp in PROFILE_KERNELS:
+        d = [x for x in dur.get(key, []) if x > 0.02]   # early-exit launches (converged problems) left out
+        fl, wl = fetch.get(key, []), write.get(key, [])
+        f = [x for x in fl if fl and x > 0.05 * max(fl)]
+        w = [x for x in wl if wl and x > 0.05 * max(wl)]
+        alg = bpp * pts
+        rec = {"symbol": sym, "alg_bytes_per_launch": alg}
+        if d:
+            rec.update(avg_ms=statistics.mean(d), launches=len(d))
+        if f and w:
+            rec.update(fetch_kib=statistics.mean(f), write_kib=statistics.mean(w),
+                       traffic_bytes_per_launch=round((statistics.mean(f) * kf + statistics.mean(w) * kw) * 1024))
+        out["kernels"][key] = rec
+        if d:
+            gbs = alg / (rec["avg_ms"] / 1e3) / 1e9
+            tr = rec.get("traffic_bytes_per_launch")
+            rows.append(f"| {label} | `{sym}` | {len(d)} | {rec['avg_ms']:.4f} | {alg / 1e6:.1f} | {gbs:.0f} | "
+                        f"{gbs / HBM_PEAK_GBS:.3f} | " + (f"{tr / 1e6:.1f} ({tr / alg:.2f}x)" if tr else "-") + " |")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "hbm_traffic_lapl_cube_127_batch32.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    md = [
+        "# LaplCube 127^3, batch 32: kernel trace and HBM traffic (MI355X)\n",
+        "Workload: 32 independent Dirichlet 127^3 problems per step (N = 128), seeded manufactured fields, "
+        "FP32+refine to 1e-10 (2 inner solves, default kernel variants) and plain FP64.\n",
+        "Commands (from the repository root, one MI355X; the last one writes this file and the json beside it):\n",
+        "```",
+        "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/kt -o kt -- python3 bench.py --steps 5 --warmup 1 --no-cpu --legs head",
+        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmcA/FETCH_SIZE -o FETCH_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
+        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmcA/WRITE_SIZE -o WRITE_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
+        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmcA/copy_FETCH_SIZE -o copy_FETCH_SIZE -- python3 bench.py --calib-copy-mib 256",
+        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmcA/copy_WRITE_SIZE -o copy_WRITE_SIZE -- python3 bench.py --calib-copy-mib 256",
+        "python3 bench.py --profile-summary gpurun_out/kt gpurun_out/pmcA",
+        "```\n",
+        f"Calibration: a 256 MiB device-to-device copy reads FETCH_SIZE = {copy_kib / kf:.0f} KiB and WRITE_SIZE = "
+        f"{copy_kib / kw:.0f} KiB, so FETCH_SIZE is scaled by {kf:.2f} and WRITE_SIZE by {kw:.2f} to get bytes.\n",
+        "Durations are per dispatch from the kernel trace.  Dispatches shorter than 20 us (launches for problems "
+        "already converged, which read one flag and exit) are left out, and so are their counter readings.\n",
+        "| kernel | symbol | dispatches | avg ms | algorithmic MB / launch | achieved GB/s | of 8 TB/s | "
+        "HBM traffic MB / launch (x alg) |",
+        "|---|---|---|---|---|---|---|---|",
+    ] + rows + ["", "rocprofv3 --stats summary (all dispatches, warm-up included):", "",
+                "| kernel | calls | avg ns | % |", "|---|---|---|---|"]
+    for r in stats:
+        md.append(f"| `{r['Name'].split('(')[0].replace('void ', '')}` | {r['Calls']} | {float(r['AverageNs']):.0f} | "
+                  f"{float(r['Percentage']):.2f} |")
+    with open(os.path.join(ROOT, "profiles", "lapl_cube_127_batch32_kernels.md"), "w") as fh:
+        fh.write("\n".join(md) + "\n")
+    print("\n".join(md))
 
 
 def measured_traffic(kernel: str, n: int, batch: int):
@@ -176,7 +291,7 @@ def measured_traffic(kernel: str, n: int, batch: int):
         return None
     if t.get("n") != n or t.get("batch") != batch:
         return None
-    k = t.get("kernels", {}).get(kernel)
+    k = t.get("kernels", {}).get(PROFILE_KEY_OF_CLASS.get(kernel, kernel))
     return None if k is None else k.get("traffic_bytes_per_launch")
 
 
@@ -528,10 +643,15 @@ def main():
     ap.add_argument("--legs", default="head,c2,c3,c4,c5,cpu",
                     help="comma list of legs: head (the value; always run), c2, c3, c4, c5, cpu")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving rate")
+    ap.add_argument("--profile-summary", nargs=2, metavar=("KT_DIR", "PMC_DIR"),
+                    help="summarise rocprofv3 csv output into profiles/ and exit (no GPU)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (calibrates the HBM "
                          "FETCH_SIZE/WRITE_SIZE counters against a copy of known size) and exit")
     args = ap.parse_args()
+    if args.profile_summary:
+        profile_summary(*args.profile_summary)
+        return
     if args.calib_copy_mib:
         from fdm_amd import hip
         nb = args.calib_copy_mib << 20
@@ -580,10 +700,11 @@ def main():
     tm = res_r["timing"]
     dom = max((k for k in tm if k != "other"), key=lambda k: tm[k][0])
     ms_total, launches = tm[dom]
-    alg = algorithmic_bytes(dom, "refine", res_r["iterations"], n, B) * args.steps
+    fused = tm.get("xy_fwd_resid", (0.0, 0))[1] > 0
+    alg = algorithmic_bytes(dom, "refine", res_r["iterations"], n, B, fused) * args.steps
     achieved = alg / (ms_total / 1e3) / 1e9
     # whole-solve algorithmic bytes: the per-class model summed over the classes that ran
-    model_bytes = sum(algorithmic_bytes(k, "refine", res_r["iterations"], n, B) for k in tm) * args.steps
+    model_bytes = sum(algorithmic_bytes(k, "refine", res_r["iterations"], n, B, fused) for k in tm) * args.steps
     err_u = float(np.linalg.norm(res_d["u0"] - us[0]) / np.linalg.norm(us[0]))
     err_r = float(np.linalg.norm(res_r["u0"] - res_d["u0"]) / np.linalg.norm(res_d["u0"]))
 

@@ -12,6 +12,28 @@
 
 namespace fdmgpu {
 
+constexpr int kXcdSlabs = 8;   // XCDs of MI355X; workgroups are dealt to them round-robin
+
+// 1-D grid of per * kXcdSlabs blocks covering `blocks` blocks of each of
+// `batch` problems: the logical block (problem, block) is dealt out in
+// contiguous slabs per XCD so neighbouring blocks of a problem, whose row
+// segments share partial cache lines, run under one L2.  False for the
+// padding blocks.
+__device__ __forceinline__ bool xcd_slab_block(const long blocks, const int batch, int& p, long& blk) {
+    const long total = blocks * batch;
+    const long per = (total + kXcdSlabs - 1) / kXcdSlabs;
+    const long logical = (blockIdx.x % kXcdSlabs) * per + blockIdx.x / kXcdSlabs;
+    if (logical >= total) return false;
+    p = (int)(logical / blocks);
+    blk = logical - (long)p * blocks;
+    return true;
+}
+
+static unsigned xcd_slab_grid(const long blocks, const int batch) {
+    const long total = blocks * batch;
+    return (unsigned)((total + kXcdSlabs - 1) / kXcdSlabs * kXcdSlabs);
+}
+
 // ---------------------------------------------------------------------------
 // z cyclic reduction (see dst_device.hpp for the algorithm)
 // ---------------------------------------------------------------------------
@@ -110,17 +132,20 @@ template <typename Ts, typename T, int Q, int GM>
 __global__ __launch_bounds__(256) void zcr_kernel(const BatchPtr<Ts> data, const long plane, const int nx,
                                                   const Ts* __restrict__ lmx, const Ts* __restrict__ lmy,
                                                   const T b, const T d0, const T gamma,
-                                                  const int* __restrict__ done) {
+                                                  const int* __restrict__ done, const long blocks,
+                                                  const int batch) {
     using Cf = ZcrCfg<T, Q, GM>;
-    if (done && done[blockIdx.y]) return;
+    int p;
+    long blk;
+    if (!xcd_slab_block(blocks, batch, p, blk) || (done && done[p])) return;
     constexpr int NQ = Cf::NQ, C = Cf::C, W = Cf::W;
     const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long wave = (blk * blockDim.x + threadIdx.x) >> 6;
     const int s = lane % W, c = lane / W;
     const long m = wave * W + s;
     const bool valid = m < plane;
     const long mm = valid ? m : 0;
-    const auto base = as_global(data(blockIdx.y) + mm);
+    const auto base = as_global(data(p) + mm);
     const int ky = (int)(mm / nx), kx = (int)(mm - (long)ky * nx);
 
     T v[C];
@@ -150,6 +175,9 @@ __global__ __launch_bounds__(256) void zcr_kernel(const BatchPtr<Ts> data, const
 // waves of a strip exchange their separator contributions through LDS (one
 // barrier).  S strips per block.  The arithmetic is the lane layout's: the
 // same levels in the same order, so results are identical.
+// Blocks go round-robin over the 8 XCDs; the logical block (problem, strip
+// group) is dealt out in contiguous slabs per XCD, so neighbouring blocks,
+// whose row segments share partial cache lines, run under one L2.
 // ---------------------------------------------------------------------------
 template <int Q>
 struct ZcrStripCfg {
@@ -163,19 +191,22 @@ struct ZcrStripCfg {
 template <typename Ts, typename T, int Q, int S>
 __global__ __launch_bounds__(64 * ZcrStripCfg<Q>::G * S) void zcr_strip_kernel(
     const BatchPtr<Ts> data, const long plane, const int nx, const Ts* __restrict__ lmx,
-    const Ts* __restrict__ lmy, const T b, const T d0, const T gamma, const int* __restrict__ done) {
+    const Ts* __restrict__ lmy, const T b, const T d0, const T gamma, const int* __restrict__ done,
+    const int blocks, const int batch) {
     using Cf = ZcrStripCfg<Q>;
     constexpr int NQ = Cf::NQ, C = Cf::C, G = Cf::G, LOGG = Cf::LOGG, LOC = Cf::LOC;
     constexpr int SYS = 64 * S;   // systems per block
     __shared__ T xch[G > 1 ? 2 * G * SYS : 1];
-    if (done && done[blockIdx.y]) return;
+    int p;
+    long blk;
+    if (!xcd_slab_block(blocks, batch, p, blk) || (done && done[p])) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c = w % G, strip = w / G;
     const int sys = strip * 64 + lane;
-    const long m = (long)blockIdx.x * SYS + sys;
+    const long m = (long)blk * SYS + sys;
     const bool valid = m < plane;
     const long mm = valid ? m : 0;
-    const auto base = as_global(data(blockIdx.y) + mm);
+    const auto base = as_global(data(p) + mm);
     const int ky = (int)(mm / nx), kx = (int)(mm - (long)ky * nx);
 
     T v[C];
@@ -318,18 +349,21 @@ template <typename Ts, typename T, int Q, int GM>
 __global__ __launch_bounds__(256) void zcrp_kernel(const BatchPtr<Ts> data, const long plane, const int nx,
                                                    const Ts* __restrict__ lmx, const Ts* __restrict__ lmy,
                                                    const T b, const T d0, const T gamma,
-                                                   const int* __restrict__ done) {
+                                                   const int* __restrict__ done, const long blocks,
+                                                   const int batch) {
     using Cf = ZcrCfg<T, Q, GM>;
-    if (done && done[blockIdx.y]) return;
+    int p;
+    long blk;
+    if (!xcd_slab_block(blocks, batch, p, blk) || (done && done[p])) return;
     constexpr int NQ = Cf::NQ, G = Cf::G, C = Cf::C, W = Cf::W, LOGG = Cf::LOGG;
     constexpr int LOC = Q - LOGG;   // levels local to a lane (all of them when G = 1)
     const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long wave = (blk * blockDim.x + threadIdx.x) >> 6;
     const int s = lane % W, c = lane / W;
     const long m = wave * W + s;
     const bool valid = m < plane;
     const long mm = valid ? m : 0;
-    const auto base = as_global(data(blockIdx.y) + mm);
+    const auto base = as_global(data(p) + mm);
     const int ky = (int)(mm / nx), kx = (int)(mm - (long)ky * nx);
 
     T v[C];
@@ -514,22 +548,22 @@ template <typename Ts, typename T, int Q, int GM>
 static hipError_t zcr_k(const ZcrArgs& a) {
     using C = ZcrCfg<T, Q, GM>;
     const long lanes = ((a.plane + C::W - 1) / C::W) * 64;
-    const unsigned blocks = (unsigned)((lanes + C::NT - 1) / C::NT);
+    const long blocks = (lanes + C::NT - 1) / C::NT;
     auto k = a.periodic ? zcrp_kernel<Ts, T, Q, GM> : zcr_kernel<Ts, T, Q, GM>;
     const BatchPtr<Ts> data{(Ts*)a.data, a.data_tab, a.bs};
-    hipLaunchKernelGGL(k, dim3(blocks, a.batch), dim3(C::NT), 0, a.stream, data, a.plane, a.nx, (const Ts*)a.lmx,
-                       (const Ts*)a.lmy, (T)a.b, (T)a.d0, (T)a.gamma, a.done);
+    hipLaunchKernelGGL(k, dim3(xcd_slab_grid(blocks, a.batch)), dim3(C::NT), 0, a.stream, data, a.plane, a.nx,
+                       (const Ts*)a.lmx, (const Ts*)a.lmy, (T)a.b, (T)a.d0, (T)a.gamma, a.done, blocks, a.batch);
     return hipGetLastError();
 }
 
 template <typename Ts, typename T, int Q, int S>
 static hipError_t zcr_strip_k(const ZcrArgs& a) {
     constexpr int SYS = 64 * S;
-    const unsigned blocks = (unsigned)((a.plane + SYS - 1) / SYS);
+    const long blocks = (a.plane + SYS - 1) / SYS;
     const BatchPtr<Ts> data{(Ts*)a.data, a.data_tab, a.bs};
-    hipLaunchKernelGGL((zcr_strip_kernel<Ts, T, Q, S>), dim3(blocks, a.batch), dim3(64 * ZcrStripCfg<Q>::G * S), 0,
-                       a.stream, data, a.plane, a.nx, (const Ts*)a.lmx, (const Ts*)a.lmy, (T)a.b, (T)a.d0,
-                       (T)a.gamma, a.done);
+    hipLaunchKernelGGL((zcr_strip_kernel<Ts, T, Q, S>), dim3(xcd_slab_grid(blocks, a.batch)),
+                       dim3(64 * ZcrStripCfg<Q>::G * S), 0, a.stream, data, a.plane, a.nx, (const Ts*)a.lmx,
+                       (const Ts*)a.lmy, (T)a.b, (T)a.d0, (T)a.gamma, a.done, blocks, a.batch);
     return hipGetLastError();
 }
 

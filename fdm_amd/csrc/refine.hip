@@ -163,32 +163,36 @@ __global__ __launch_bounds__(TX* TY) void residual_tile_kernel(
     const int wcol = tx == 0 ? 0 : TX + 1;
     for (int i = threadIdx.x; i < 2 * (TY + 2) * (TX + 2); i += NT) (&t[0][0][0])[i] = 0.0;
 
+    // Values in flight stay in their storage type Tu and are widened where
+    // they are used: a conversion next to the load would make the wave wait
+    // for it there.
     long o = (long)z0 * pl;
     double um = (valid && z0 > 0) ? (double)up[o - pl + c0] : 0.0;
     double uc = valid ? (double)up[o + c0] : 0.0;
-    double un = (valid && z0 + 1 < nz) ? (double)up[o + pl + c0] : 0.0;
+    Tu un = (valid && z0 + 1 < nz) ? up[o + pl + c0] : Tu(0);
     double fv = valid ? (double)fp[o + c0] : 0.0;
-    double hn = hN ? (double)up[o + cN] : 0.0, hs = hS ? (double)up[o + cS] : 0.0;
-    double hw = hW ? (double)up[o + cW] : 0.0;
+    Tu hn = hN ? up[o + cN] : Tu(0), hs = hS ? up[o + cS] : Tu(0);
+    Tu hw = hW ? up[o + cW] : Tu(0);
     double rr = 0.0, ff = 0.0;
     __syncthreads();
     for (int z = z0; z < z1; ++z, o += pl) {
         auto& b = t[z & 1];
         b[ty + 1][tx + 1] = uc;
-        if (hN) b[0][tx + 1] = hn;
-        if (hS) b[TY + 1][tx + 1] = hs;
-        if (hW) b[ty + 1][wcol] = hw;
+        if (hN) b[0][tx + 1] = (double)hn;
+        if (hS) b[TY + 1][tx + 1] = (double)hs;
+        if (hW) b[ty + 1][wcol] = (double)hw;
         // in flight while plane z is computed: u at z+2 (plane z+1's upper
         // neighbour), f and the halos at z+1
         const bool more = z + 1 < z1;
-        const double unn = (valid && more && z + 2 < nz) ? (double)up[o + 2 * pl + c0] : 0.0;
+        const Tu unn = (valid && more && z + 2 < nz) ? up[o + 2 * pl + c0] : Tu(0);
         const double fn = (valid && more) ? (double)fp[o + pl + c0] : 0.0;
-        const double hn2 = (hN && more) ? (double)up[o + pl + cN] : 0.0;
-        const double hs2 = (hS && more) ? (double)up[o + pl + cS] : 0.0;
-        const double hw2 = (hW && more) ? (double)up[o + pl + cW] : 0.0;
+        const Tu hn2 = (hN && more) ? up[o + pl + cN] : Tu(0);
+        const Tu hs2 = (hS && more) ? up[o + pl + cS] : Tu(0);
+        const Tu hw2 = (hW && more) ? up[o + pl + cW] : Tu(0);
         __syncthreads();   // b complete; the other buffer's readers (plane z-1) are past this point
+        const double udn = (double)un;
         const double au = (b[ty + 1][tx] - 2.0 * uc + b[ty + 1][tx + 2]) * ix +
-                          (b[ty][tx + 1] - 2.0 * uc + b[ty + 2][tx + 1]) * iy + (um - 2.0 * uc + un) * iz;
+                          (b[ty][tx + 1] - 2.0 * uc + b[ty + 2][tx + 1]) * iy + (um - 2.0 * uc + udn) * iz;
         const double r = fv - au;
         if (valid) {
             rr += r * r;
@@ -210,7 +214,7 @@ __global__ __launch_bounds__(TX* TY) void residual_tile_kernel(
             ff = 0.0;
         }
         um = uc;
-        uc = un;
+        uc = udn;
         un = unn;
         fv = fn;
         hn = hn2;
