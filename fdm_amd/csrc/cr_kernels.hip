@@ -179,21 +179,21 @@ __global__ __launch_bounds__(256) void zcr_kernel(const BatchPtr<Ts> data, const
 // group) is dealt out in contiguous slabs per XCD, so neighbouring blocks,
 // whose row segments share partial cache lines, run under one L2.
 // ---------------------------------------------------------------------------
-template <int Q>
+template <int Q, int CR = 32>
 struct ZcrStripCfg {
     static constexpr int NQ = 1 << Q;
-    static constexpr int C = NQ >= 32 ? 32 : NQ;   // rows per lane (incl. separator slot)
+    static constexpr int C = NQ >= CR ? CR : NQ;   // rows per lane (incl. separator slot)
     static constexpr int G = NQ / C;               // chunk waves per strip
     static constexpr int LOGG = ilog2(G);
     static constexpr int LOC = G == 1 ? Q - 1 : Q - LOGG;
 };
 
-template <typename Ts, typename T, int Q, int S>
-__global__ __launch_bounds__(64 * ZcrStripCfg<Q>::G * S) void zcr_strip_kernel(
+template <typename Ts, typename T, int Q, int S, int CR>
+__global__ __launch_bounds__((64 * ZcrStripCfg<Q, CR>::G * S)) void zcr_strip_kernel(
     const BatchPtr<Ts> data, const long plane, const int nx, const Ts* __restrict__ lmx,
     const Ts* __restrict__ lmy, const T b, const T d0, const T gamma, const int* __restrict__ done,
     const int blocks, const int batch) {
-    using Cf = ZcrStripCfg<Q>;
+    using Cf = ZcrStripCfg<Q, CR>;
     constexpr int NQ = Cf::NQ, C = Cf::C, G = Cf::G, LOGG = Cf::LOGG, LOC = Cf::LOC;
     constexpr int SYS = 64 * S;   // systems per block
     __shared__ T xch[G > 1 ? 2 * G * SYS : 1];
@@ -533,11 +533,13 @@ __global__ __launch_bounds__(256) void rowcr_kernel(const BatchPtr<Ts> data, con
 // Lane multipliers a z system of 2^q rows supports: 1 always, 2 when every
 // lane still holds at least 2 rows; the strip layouts (3: 64 systems per
 // block, 4: 128) from 32 rows up (Dirichlet only), 4 up to 128 rows (512
-// threads, registers stay within budget in FP64).
+// threads, registers stay within budget in FP64); 5: 16-row chunks (half the
+// registers per lane), 16 to 256 rows.
 bool zcr_gmul_ok(int q, int gm) {
     if (gm == 1) return true;
     if (gm == 3) return q >= 5 && q <= 9;
     if (gm == 4) return q >= 5 && q <= 7;
+    if (gm == 5) return q >= 4 && q <= 8;
     if (gm != 2 || q < 2) return false;
     const int NQ = 1 << q;
     const int G = (NQ > 64 ? NQ / 64 : 1) * gm;
@@ -556,28 +558,35 @@ static hipError_t zcr_k(const ZcrArgs& a) {
     return hipGetLastError();
 }
 
-template <typename Ts, typename T, int Q, int S>
+template <typename Ts, typename T, int Q, int S, int CR>
 static hipError_t zcr_strip_k(const ZcrArgs& a) {
     constexpr int SYS = 64 * S;
     const long blocks = (a.plane + SYS - 1) / SYS;
     const BatchPtr<Ts> data{(Ts*)a.data, a.data_tab, a.bs};
-    hipLaunchKernelGGL((zcr_strip_kernel<Ts, T, Q, S>), dim3(xcd_slab_grid(blocks, a.batch)),
-                       dim3(64 * ZcrStripCfg<Q>::G * S), 0, a.stream, data, a.plane, a.nx, (const Ts*)a.lmx,
+    hipLaunchKernelGGL((zcr_strip_kernel<Ts, T, Q, S, CR>), dim3(xcd_slab_grid(blocks, a.batch)),
+                       dim3(64 * ZcrStripCfg<Q, CR>::G * S), 0, a.stream, data, a.plane, a.nx, (const Ts*)a.lmx,
                        (const Ts*)a.lmy, (T)a.b, (T)a.d0, (T)a.gamma, a.done, blocks, a.batch);
     return hipGetLastError();
 }
 
 template <typename Ts, typename T, int Q>
 static hipError_t zcr_g(const ZcrArgs& a) {
-    // default: Dirichlet systems of 32 to 512 rows in the strip layout (64
-    // consecutive modes per wave; measured fastest at 128 rows), circulant ones
+    // default: Dirichlet systems in the strip layout (64 consecutive modes per
+    // wave) with 16-row chunks (16 to 256 rows; measured fastest at 128 rows:
+    // 8 waves per SIMD in FP32), 512 rows with 32-row chunks; circulant ones
     // with 2 lanes per 64 rows from 64 rows up (fewer registers per lane,
     // twice the occupancy of 1)
-    const int gm = a.gmul ? a.gmul : (!a.periodic && zcr_gmul_ok(Q, 3)) ? 3 : (Q >= 6 ? 2 : 1);
+    const int gm = a.gmul                                   ? a.gmul
+                   : (!a.periodic && zcr_gmul_ok(Q, 5)) ? 5
+                   : (!a.periodic && zcr_gmul_ok(Q, 3)) ? 3
+                   : (Q >= 6 ? 2 : 1);
     if (!zcr_gmul_ok(Q, gm) || (gm >= 3 && a.periodic)) return hipErrorInvalidValue;
-    if (gm == 3) return zcr_strip_k<Ts, T, Q, 1>(a);
+    if (gm == 3) return zcr_strip_k<Ts, T, Q, 1, 32>(a);
     if constexpr (ZcrStripCfg<Q>::G <= 4) {
-        if (gm == 4) return zcr_strip_k<Ts, T, Q, 2>(a);
+        if (gm == 4) return zcr_strip_k<Ts, T, Q, 2, 32>(a);
+    }
+    if constexpr (ZcrStripCfg<Q, 16>::G <= 16) {
+        if (gm == 5) return zcr_strip_k<Ts, T, Q, 1, 16>(a);
     }
     if (gm == 1) return zcr_k<Ts, T, Q, 1>(a);
     if constexpr (Q >= 2) {

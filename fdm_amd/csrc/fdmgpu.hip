@@ -214,7 +214,7 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
     std::vector<int> gms{1};
     if (P->kind == 3 && zcr_gmul_ok(P->logz, 2)) gms.push_back(2);
     if (P->kind == 3 && !(P->periodic & FDMGPU_PERIODIC_Z))
-        for (int gm : {3, 4})
+        for (int gm : {3, 4, 5})
             if (zcr_gmul_ok(P->logz, gm)) gms.push_back(gm);
     std::vector<Variant> out;
     Variant d;

@@ -103,8 +103,9 @@ int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_ta
  * kernels (16, 32, 64; 0 = default); zcr_gmul = z cyclic reduction
  * layout: 1, 2 = lanes per system multiplier inside a wavefront, 3, 4 = one
  * wavefront per 32-row chunk of 64 consecutive modes, 64 (3) or 128 (4)
- * modes per block (Dirichlet z, 32 to 512 rows; 4 up to 128); 0 = default
- * (3 where valid, else 2 from 64 rows, else 1); fuse (FP32+refine plane
+ * modes per block (Dirichlet z, 32 to 512 rows; 4 up to 128), 5 = the same
+ * with 16-row chunks (16 to 256 rows); 0 = default
+ * (5, else 3 where valid, else 2 from 64 rows, else 1); fuse (FP32+refine plane
  * path) = FP64 residual fused into the next forward transform (1) or a
  * separate residual pass (0, the default).  Every variant solves the same system; results
  * differ at rounding level.  No reference counterpart. */
