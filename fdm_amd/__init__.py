@@ -30,7 +30,7 @@ OK, ERR_INVALID_SIZE, ERR_INVALID_ARG, ERR_UNSUPPORTED = 0, 1, 2, 3
 ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_ADMISSION = 4, 5, 6, 7, 8
 
 # every entry point include/fdmgpu.h declares
-EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_rect_set_scales", "fdmgpu_set_refine",
+EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdmgpu_rect_set_scales", "fdmgpu_set_refine",
            "fdmgpu_solve_host", "fdmgpu_solve_device", "fdmgpu_solve_batch_device", "fdmgpu_solve_table_device",
            "fdmgpu_plan_variants", "fdmgpu_plan_set_variant", "fdmgpu_plan_get_variant", "fdmgpu_get_stats",
            "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_group_create",
@@ -103,6 +103,7 @@ def _lib_handle():
         vp, d, i = C.c_void_p, C.c_double, C.c_int
         L.fdmgpu_plan_cube.argtypes = [C.POINTER(vp), i, i, d, d, d, d, d, d, i, i, i]
         L.fdmgpu_plan_rect.argtypes = [C.POINTER(vp), i, i, d, d, d, d, i, i]
+        L.fdmgpu_plan_rect_fft2.argtypes = [C.POINTER(vp), i, i, d, d, d, d, i, i]
         L.fdmgpu_rect_set_scales.argtypes = [vp, vp, vp, vp]
         L.fdmgpu_set_refine.argtypes = [vp, d, i]
         L.fdmgpu_solve_host.argtypes = [vp, vp, vp]
@@ -361,6 +362,23 @@ class LaplRect(_Plan):
 
         _check(_lib_handle().fdmgpu_rect_set_scales(self._h, ptr(lm_y_scale), ptr(L_scale), ptr(U_scale)),
                "LaplRect.set_scales")
+
+
+class LaplRectFFT2(LaplRect):
+    """LaplRectFFT2<T,check,F>(dx,dy,lx,ly,nx,ny) (reference src/lapl_rect.h:76-109):
+    transforms along both axes.  yperiodic = tensor_flags<periodic>; both
+    = tensor_flags<periodic, periodic> (zero mode coefficient 1).  Arrays are
+    [ny][nx] over the reference's index ranges (all n points on a periodic
+    axis).  set_scales(lm_y_scale=...) as the reference's public member."""
+    _kind = "LaplRectFFT2"
+
+    def __init__(self, dx, dy, lx, ly, nx, ny, precision="f64", yperiodic=False, xperiodic=False):
+        h = C.c_void_p()
+        p = _prec(precision)
+        mask = (PERIODIC_Y if yperiodic else 0) | (PERIODIC_X if xperiodic else 0)
+        _check(_lib_handle().fdmgpu_plan_rect_fft2(C.byref(h), p, mask, dx, dy, lx, ly, nx, ny), "LaplRectFFT2")
+        _Plan.__init__(self, h, p, (ny, nx))
+        self.dx, self.dy, self.lx, self.ly, self.nx, self.ny = dx, dy, lx, ly, nx, ny
 
 
 NS_FIELDS = ("u", "v", "w", "p", "x", "F", "G", "H", "RHS")

@@ -640,6 +640,40 @@ hipError_t launch_rowcr(const RowCrArgs& a) {
     return rowcr_k<T, T>(a);
 }
 
+// ---------------------------------------------------------------------------
+// LaplRectFFT2 (src/lapl_rect.cpp:162-172): after the transforms along y and
+// x every coefficient is divided by -(lm_y[k] lm_y_scale[j] + lm_x[j]), in T
+// as the reference does; the double-periodic plan sets the (0,0) coefficient
+// (0/0) to 1.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void rect2_divide_kernel(const BatchPtr<T> data, const int nx, const int ny,
+                                                          const T* __restrict__ lmy, const T* __restrict__ lmx,
+                                                          const T* __restrict__ sc, const bool zero_mode,
+                                                          const int* __restrict__ done) {
+    if (done && done[blockIdx.y]) return;
+    const long n = (long)nx * ny;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = (int)(i / nx), j = (int)(i - (long)k * nx);
+    const auto d = as_global(data(blockIdx.y));
+    T v = d[i];
+    v /= -lmy[k] * sc[j] - lmx[j];
+    if (zero_mode && i == 0) v = T(1);
+    d[i] = v;
+}
+
+template <typename T>
+hipError_t launch_rect2_divide(const Rect2DivArgs& a) {
+    const long n = (long)a.nx * a.ny;
+    const BatchPtr<T> data{(T*)a.data, a.data_tab, a.bs};
+    hipLaunchKernelGGL(rect2_divide_kernel<T>, dim3((unsigned)((n + 255) / 256), a.batch), dim3(256), 0, a.stream,
+                       data, a.nx, a.ny, (const T*)a.lmy, (const T*)a.lmx, (const T*)a.sc, a.zero_mode, a.done);
+    return hipGetLastError();
+}
+
+template hipError_t launch_rect2_divide<float>(const Rect2DivArgs&);
+template hipError_t launch_rect2_divide<double>(const Rect2DivArgs&);
 template hipError_t launch_zcr<float>(int, const ZcrArgs&);
 template hipError_t launch_zcr<double>(int, const ZcrArgs&);
 template hipError_t launch_rowcr<float>(const RowCrArgs&);

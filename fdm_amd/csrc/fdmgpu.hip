@@ -231,7 +231,7 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
                 out.push_back(v);
             }
         }
-        for (int tx : (P->kind == 3 ? tiles(P->logx) : std::vector<int>{0}))
+        for (int tx : (P->kind == 3 || P->fft2 ? tiles(P->logx) : std::vector<int>{0}))
             for (int ty : tiles(P->logy)) {
                 Variant v;
                 v.plane = 0;
@@ -321,11 +321,19 @@ int stage_fwd(fdmgpu_plan_t* P, const void* in, Tab in_tab, void* out, Tab out_t
         HIPCHK((launch_yaxis<T, T, T>(P->logy, ay)));
         return FDMGPU_OK;
     }
-    // rect: y only
+    // rect: y (LaplRectFFT2: then x)
     AxisArgs ay{in, out, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd, in_tab, out_tab,
                 P->var.tile_y};
-    Timed tm(P, FDMGPU_K_Y_TILE, s);
-    HIPCHK((launch_yaxis<Tin, T, T>(P->logy, ay)));
+    {
+        Timed tm(P, FDMGPU_K_Y_TILE, s);
+        HIPCHK((launch_yaxis<Tin, T, T>(P->logy, ay)));
+    }
+    if (P->fft2) {
+        AxisArgs ax{out, out, (long)P->ny, P->nx, 1, elems, elems, count, sx, P->tab_x, s, false, done, P->kxf,
+                    out_tab, out_tab, P->var.tile_x};
+        Timed tm(P, FDMGPU_K_X_TILE, s);
+        HIPCHK((launch_xaxis<T, T, T>(P->logx, ax)));
+    }
     return FDMGPU_OK;
 }
 
@@ -357,6 +365,12 @@ int stage_inv(fdmgpu_plan_t* P, void* in, Tab in_tab, void* out, Tab out_tab, in
         HIPCHK((launch_xaxis<T, Tout, T>(P->logx, ax)));
         return FDMGPU_OK;
     }
+    if (P->fft2) {   // LaplRectFFT2: inverse x in place first, then y as LaplRect
+        AxisArgs ax{in, in, (long)P->ny, P->nx, 1, elems, elems, count, sx, P->tab_x, s, false, done, P->kxi, in_tab,
+                    in_tab, P->var.tile_x};
+        Timed tm(P, FDMGPU_K_X_TILE, s);
+        HIPCHK((launch_xaxis<T, T, T>(P->logx, ax)));
+    }
     if (accum) {   // rect: inverse y into the FP32 buffer, then accumulate through an x pass-free add
         AxisArgs ay{in, in, 0, P->nx, 1, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv, in_tab, in_tab,
                     P->var.tile_y};
@@ -387,6 +401,13 @@ int stage_tridiag(fdmgpu_plan_t* P, void* data, Tab data_tab, int count, hipStre
                   P->precision == FDMGPU_F32, data_tab, P->var.zcr_gmul};
         Timed tm(P, FDMGPU_K_ZCR, s);
         HIPCHK(launch_zcr<T>(P->logz, z));
+        return FDMGPU_OK;
+    }
+    if (P->fft2) {
+        const bool zm = (P->periodic & FDMGPU_PERIODIC_X) && (P->periodic & FDMGPU_PERIODIC_Y);
+        Rect2DivArgs d{data, P->nx, P->ny, P->lmy, P->lmx, P->d_sc, zm, elems, count, s, done, data_tab};
+        Timed tm(P, FDMGPU_K_ROWCR, s);
+        HIPCHK(launch_rect2_divide<T>(d));
         return FDMGPU_OK;
     }
     RowCrArgs r{data, P->nx, P->ny, P->lmy, P->d_sc, P->d_Ls, P->d_Us, 1.0 / (P->dx * P->dx), elems, count, s, done,
@@ -600,7 +621,33 @@ void free_plan(fdmgpu_plan_t* P) {
     delete P;
 }
 
+int set_scales_fft2(fdmgpu_plan_t* P, const double* s) {
+    const int nx = P->nx;
+    const bool xper = (P->periodic & FDMGPU_PERIODIC_X) != 0, yper = (P->periodic & FDMGPU_PERIODIC_Y) != 0;
+    const int x1 = xper ? 0 : 1;
+    if (s && P->precision == FDMGPU_F32_REFINE64) return FDMGPU_ERR_UNSUPPORTED;   // no real-space stencil
+    P->h_sc.assign(nx, 1.0);
+    for (int j = 0; j < nx; j++) {
+        if (s) P->h_sc[j] = s[j + x1];
+        if (P->precision == FDMGPU_F32) P->h_sc[j] = (double)(float)P->h_sc[j];
+    }
+    // every mode's divisor must be nonzero (the double-periodic zero mode is replaced)
+    const std::vector<double> ly = yper ? periodic_lambda(P->ny, P->dy, false) : dirichlet_lambda(P->ny, P->dy, false);
+    const std::vector<double> lx = P->h_lmx;
+    for (int k = 0; k < P->ny; k++)
+        for (int j = 0; j < nx; j++)
+            if (!(k == 0 && j == 0 && xper && yper) && ly[k] * P->h_sc[j] + lx[j] == 0.0) {
+                std::snprintf(g_errbuf, sizeof g_errbuf, "LaplRectFFT2 mode (%d, %d) has a zero eigenvalue", k, j);
+                return FDMGPU_ERR_SINGULAR;
+            }
+    const bool f32 = P->precision != FDMGPU_F64;
+    int rc;
+    if ((rc = f32 ? upload<float>(&P->d_sc, P->h_sc) : upload<double>(&P->d_sc, P->h_sc))) return rc;
+    return upload<double>((void**)&P->d_sc64, P->h_sc);
+}
+
 int set_scales_impl(fdmgpu_plan_t* P, const double* s, const double* L, const double* U) {
+    if (P->fft2) return set_scales_fft2(P, s);
     const int nx = P->nx;
     P->h_sc.assign(nx, 1.0);
     P->h_Ls.assign(nx, 1.0);
@@ -756,6 +803,55 @@ int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double
         return rc;
     }
     P->var.plane = plane_capable(P) ? 1 : 0;   // the default variant
+    apply_tuned(P);
+    *plan = P;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_plan_rect_fft2(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double lx,
+                          double ly, int nx, int ny) {
+    if (!plan || !valid_precision(precision)) return FDMGPU_ERR_INVALID_ARG;
+    *plan = nullptr;
+    if (periodic_mask & ~(FDMGPU_PERIODIC_X | FDMGPU_PERIODIC_Y)) return FDMGPU_ERR_INVALID_ARG;
+    if (nx < 1 || ny < 1) return FDMGPU_ERR_INVALID_SIZE;
+    const bool yper = periodic_mask & FDMGPU_PERIODIC_Y, xper = periodic_mask & FDMGPU_PERIODIC_X;
+    const int lgy = ilog2_exact(yper ? ny : ny + 1), lgx = ilog2_exact(xper ? nx : nx + 1);
+    if (lgy < 0 || lgx < 0) return FDMGPU_ERR_INVALID_SIZE;
+    if (lgy < 1 || lgy > 9 || lgx < 1 || lgx > 9) return FDMGPU_ERR_UNSUPPORTED;
+    if (precision == FDMGPU_F32_REFINE64 &&
+        (periodic_mask || std::fabs(dy * (ny + 1) / ly - 1.0) > 1e-13 || std::fabs(dx * (nx + 1) / lx - 1.0) > 1e-13))
+        return FDMGPU_ERR_UNSUPPORTED;   // the residual stencil is the transforms' operator only then
+    fdmgpu_plan_t* P = new (std::nothrow) fdmgpu_plan_t();
+    if (!P) return FDMGPU_ERR_OUT_OF_MEMORY;
+    int rc = common_init(P, precision);
+    if (rc) { free_plan(P); return rc; }
+    P->kind = 2;
+    P->fft2 = true;
+    P->periodic = periodic_mask;
+    P->kfwd = yper ? kRdftFwd : kDst;
+    P->kinv = yper ? kRdftInv : kDst;
+    P->kxf = xper ? kRdftFwd : kDst;
+    P->kxi = xper ? kRdftInv : kDst;
+    P->nx = nx; P->ny = ny; P->nz = 1;
+    P->logx = lgx; P->logy = lgy; P->logz = 0;
+    P->dx = dx; P->dy = dy; P->lx = lx; P->ly = ly;
+    const bool as_float = precision == FDMGPU_F32;
+    std::vector<double> lmy = yper ? periodic_lambda(ny, dy, as_float) : dirichlet_lambda(ny, dy, as_float);
+    std::vector<double> lmx = xper ? periodic_lambda(nx, dx, as_float) : dirichlet_lambda(nx, dx, as_float);
+    // lm_x aliases lm_y in the all-Dirichlet instantiation when the point
+    // counts match (src/lapl_rect.cpp:36-40)
+    const bool alias = !periodic_mask && nx == ny;
+    if (alias) lmx = lmy;
+    P->hx_op = alias ? dy : dx;
+    P->h_lmx = lmx;
+    const bool f32 = precision != FDMGPU_F64;
+    if ((rc = alloc_tables(P)) || (rc = f32 ? upload<float>(&P->lmy, lmy) : upload<double>(&P->lmy, lmy)) ||
+        (rc = f32 ? upload<float>(&P->lmx, lmx) : upload<double>(&P->lmx, lmx)) ||
+        (rc = set_scales_impl(P, nullptr, nullptr, nullptr))) {
+        free_plan(P);
+        return rc;
+    }
+    P->var.plane = 0;
     apply_tuned(P);
     *plan = P;
     return FDMGPU_OK;

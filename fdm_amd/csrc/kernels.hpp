@@ -101,6 +101,22 @@ struct RowCrArgs {
     const void* const* data_tab;
 };
 
+// LaplRectFFT2 spectral solve: c[k][j] /= -(lmy[k] * sc[j] + lmx[j]) in the
+// compute type; with `zero_mode` the (0,0) coefficient becomes 1.
+struct Rect2DivArgs {
+    void* data;
+    int nx, ny;
+    const void* lmy;      // per y mode (compute type)
+    const void* lmx;      // per x mode
+    const void* sc;       // lm_y_scale per x mode
+    bool zero_mode;
+    long bs;
+    int batch;
+    hipStream_t stream;
+    const int* done;
+    const void* const* data_tab;
+};
+
 // Forward plane DST of r = f - A u (FP64 residual fused into the load) to FP32,
 // with per-plane (sum r^2, sum f^2) partials for the convergence check.
 struct PlaneResidArgs {
@@ -138,5 +154,7 @@ int default_tile(int logn);
 bool zcr_gmul_ok(int q, int gm);
 template <typename T>
 hipError_t launch_rowcr(const RowCrArgs& a);
+template <typename T>
+hipError_t launch_rect2_divide(const Rect2DivArgs& a);
 
 }  // namespace fdmgpu

@@ -48,6 +48,8 @@ struct fdmgpu_plan_t {
     int precision = FDMGPU_F64;
     int periodic = 0;
     int kfwd = 0, kinv = 0;             // pencil transform kinds (kDst, or kRdftFwd/kRdftInv if periodic)
+    bool fft2 = false;                  // LaplRectFFT2: x transformed too (kinds kxf/kxi), spectral divide
+    int kxf = 0, kxi = 0;
     int nx = 0, ny = 0, nz = 1;
     int logx = 0, logy = 0, logz = 0;   // log2 of the transform lengths
     double dx = 1, dy = 1, dz = 1, lx = 1, ly = 1, lz = 1;
@@ -61,6 +63,7 @@ struct fdmgpu_plan_t {
     void* lmy = nullptr;
     // LaplRect scales, per column (index 0..nx-1 = reference index 1..nx)
     std::vector<double> h_sc, h_Ls, h_Us;
+    std::vector<double> h_lmx;          // LaplRectFFT2: x eigenvalues per mode
     void* d_sc = nullptr;               // compute type
     void* d_Ls = nullptr;
     void* d_Us = nullptr;

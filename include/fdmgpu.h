@@ -63,6 +63,20 @@ int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double
 int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double lx,
                      double ly, int nx, int ny);
 
+/* Replaces LaplRectFFT2<T,check,F>::LaplRectFFT2(dx,dy,lx,ly,nx,ny)
+ * (src/lapl_rect.h:76-109; solve src/lapl_rect.cpp:113-206): sine or real
+ * DFT transforms along both axes and a division by the eigenvalues.
+ * periodic_mask: 0 (tensor_flags<>), FDMGPU_PERIODIC_Y (tensor_flags<periodic>)
+ * or FDMGPU_PERIODIC_Y | FDMGPU_PERIODIC_X (periodic^2, whose zero mode gets
+ * the reference's coefficient 1, src/lapl_rect.cpp:169-172); an x-periodic,
+ * y-Dirichlet plan is accepted too.  ny+1 / ny and nx+1 / nx (Dirichlet /
+ * periodic axis) must be powers of two up to 512.  FDMGPU_F32_REFINE64 only
+ * for Dirichlet axes with l = (n+1) h.  fdmgpu_rect_set_scales sets
+ * lm_y_scale (indexed by the x mode, entries x1..xn as the reference reads
+ * them); L_scale and U_scale do not enter this solver. */
+int fdmgpu_plan_rect_fft2(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double lx,
+                          double ly, int nx, int ny);
+
 /* Replaces the public LaplRect members lm_y_scale, L_scale, U_scale
  * (src/lapl_rect.h:32, 57-59): arrays of nx+1 doubles indexed 1..nx (entry 0
  * unused, as in the reference); NULL keeps the default of all ones.  Returns

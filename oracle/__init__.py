@@ -215,8 +215,10 @@ def lapl_rect_fft2(rhs: np.ndarray, dx, dy, lx, ly, yperiodic=False, xperiodic=F
     r = _c(rhs, np.float64)
     ans = np.zeros_like(r)
     keep = []
-    lib().orc_lapl_rect_fft2_d(ans, r, int(yperiodic), int(xperiodic), dx, dy, lx, ly, nx, ny,
-                              _scales_ptr(lm_y_scale, np.float64, keep))
+    rc = lib().orc_lapl_rect_fft2_d(ans, r, int(yperiodic), int(xperiodic), dx, dy, lx, ly, nx, ny,
+                                   _scales_ptr(lm_y_scale, np.float64, keep))
+    if rc:
+        raise ValueError("LaplRectFFT2: transform lengths must be powers of two")
     return ans
 
 

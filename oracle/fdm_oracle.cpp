@@ -789,7 +789,7 @@ int orc_lapl_rect_f(float* ans, const float* rhs, int yper, double dx, double dy
                     int nx, int ny, const float* lm_y_scale, const float* L_scale, const float* U_scale) {
     return lapl_rect_solve<float>(ans, rhs, yper, dx, dy, lx, ly, nx, ny, lm_y_scale, L_scale, U_scale);
 }
-// PINNED-BY: test_rect_vs_fft2, test_fft2_dense.
+// PINNED-BY: test_rect_vs_fft2, test_fft2_dense, test_fft2_periodic_operator.
 int orc_lapl_rect_fft2_d(double* ans, const double* rhs, int yper, int xper, double dx, double dy,
                          double lx, double ly, int nx, int ny, const double* lm_y_scale) {
     return lapl_rect_fft2_solve<double>(ans, rhs, yper, xper, dx, dy, lx, ly, nx, ny, lm_y_scale);

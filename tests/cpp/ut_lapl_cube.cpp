@@ -99,6 +99,27 @@ double rect_biquadratic(int n) {
     return err / nrm;
 }
 
+// LaplRect (transform + tridiagonal) against LaplRectFFT2 (two transforms) on
+// the analytic rhs of ut/ut_lapl_rect.cpp:384-456 at 511^2; the reference
+// asserts agreement to 1e-15 absolute (its answers are O(1e-1)).
+double rect_fft1_fft2(int n) {
+    const double h = 1.0 / n;
+    fdm::LaplRect<double, true> lapl1(h, h, 1 + h, 1 + h, n, n);
+    fdm::LaplRectFFT2<double, true> lapl2(h, h, 1 + h, 1 + h, n, n);
+    std::vector<double> rhs((size_t)n * n), a1(rhs.size()), a2(rhs.size());
+    for (int k = 1; k <= n; k++)
+        for (int j = 1; j <= n; j++) {
+            const double x = h * j - h / 2, y = h * k - h / 2;
+            rhs[(size_t)(k - 1) * n + (j - 1)] = 2 * std::sin(y) * std::sin(y) - 2 * std::cos(y) * std::cos(y) -
+                                                 2 * std::sin(x) * std::sin(x) + 2 * std::cos(x) * std::cos(x);
+        }
+    lapl1.solve(a1.data(), rhs.data());
+    lapl2.solve(a2.data(), rhs.data());
+    double err = 0;
+    for (size_t i = 0; i < a1.size(); i++) err = std::max(err, std::fabs(a1[i] - a2[i]));
+    return err;
+}
+
 // periodic^3 on [0, 2pi)^3, n = 32: u = sin^3 x + cos^5 y - sin z with its
 // analytic Laplacian as rhs; the solve's mean is removed and the max error
 // must stay below 1e-2 (ut/ut_lapl_cube.cpp:134-244).
@@ -156,6 +177,10 @@ int main() {
     e = rect_biquadratic<float>(31);
     std::printf("LaplRect<float> 31^2 biquadratic: err %.3e\n", e);
     CHECK(e < 1e-5, "rect float err %g", e);
+
+    e = rect_fft1_fft2(511);
+    std::printf("LaplRect vs LaplRectFFT2<double> 511^2: max abs diff %.3e\n", e);
+    CHECK(e < 1e-13, "rect fft1/fft2 diff %g", e);
 
     // the reference aborts for a non-power-of-two transform length (src/fft.cpp:67);
     // with FDMGPU_THROW the shim throws instead

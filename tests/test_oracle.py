@@ -358,6 +358,88 @@ def test_rect_vs_fft2():
     assert np.max(np.abs(a1 - a2)) < 1e-13
 
 
+def _fft2_matrices(points, periodic, h, l):
+    """Forward / inverse 1-D transforms of LaplRectFFT2 as dense matrices,
+    from the series the reference's sFFT / pFFT_1 / pFFT compute
+    (src/fft.cpp:108-212, 293-365): DST-I on the N-1 interior points of a
+    Dirichlet axis, the half-complex real DFT on the N points of a periodic one;
+    factor h*sqrt(2/l) forward, sqrt(2/l) inverse."""
+    N = points
+    sl = np.sqrt(2.0 / l)
+    if not periodic:
+        k = np.arange(1, N)
+        S = np.sin(np.pi * np.outer(k, k) / N)
+        return h * sl * S, sl * S
+    j = np.arange(N)
+    R = np.zeros((N, N))
+    Q = np.zeros((N, N))
+    R[0] = 1.0
+    R[N // 2] = (-1.0) ** j
+    Q[:, 0] = 0.5
+    Q[:, N // 2] = 0.5 * (-1.0) ** j
+    for k in range(1, N // 2):
+        R[k] = np.cos(2 * np.pi * k * j / N)
+        R[N - k] = np.sin(2 * np.pi * k * j / N)
+        Q[:, k] = np.cos(2 * np.pi * k * j / N)
+        Q[:, N - k] = np.sin(2 * np.pi * k * j / N)
+    return h * sl * R, sl * Q
+
+
+@pytest.mark.parametrize("ny,nx,yper,xper", [(7, 15, False, False), (15, 15, False, False), (8, 7, True, False),
+                                             (7, 16, False, True), (8, 16, True, True)])
+def test_fft2_dense(ny, nx, yper, xper):
+    # LaplRectFFT2::solve (src/lapl_rect.cpp:113-206) as matrix products, with
+    # init_lm (:25-41): lm_x aliases lm_y (spacing dy) in the all-Dirichlet
+    # instantiation when the point counts match; a spectral lm_y_scale; the
+    # double-periodic zero-mode coefficient 1 (:169-172).
+    dx, dy = 0.3, 0.23
+    ypoints, xpoints = (ny if yper else ny + 1), (nx if xper else nx + 1)
+    lx, ly = xpoints * dx, ypoints * dy
+    rhs = O.field(21, (ny, nx))
+    scale = np.r_[1.3, 0.7 + 0.05 * np.arange(nx)]     # nx+1 entries, x1..xn read
+    Tyf, Tyi = _fft2_matrices(ypoints, yper, dy, ly)
+    Txf, Txi = _fft2_matrices(xpoints, xper, dx, lx)
+    ky = np.arange(0, ny) if yper else np.arange(1, ny + 1)
+    kx = np.arange(0, nx) if xper else np.arange(1, nx + 1)
+    lmy = 4 / dy ** 2 * np.sin(ky * np.pi / ny if yper else ky * np.pi * 0.5 / (ny + 1)) ** 2
+    lmx = 4 / dx ** 2 * np.sin(kx * np.pi / nx if xper else kx * np.pi * 0.5 / (nx + 1)) ** 2
+    if not yper and not xper and xpoints == ypoints:
+        lmx = lmy
+    sc = scale[(0 if xper else 1):][:nx]
+    W = Tyf @ rhs @ Txf.T
+    with np.errstate(divide="ignore", invalid="ignore"):
+        W = W / (-(lmy[:, None] * sc[None, :] + lmx[None, :]))
+    if yper and xper:
+        W[0, 0] = 1.0
+    want = Tyi @ W @ Txi.T
+    got = O.lapl_rect_fft2(rhs, dx, dy, lx, ly, yperiodic=yper, xperiodic=xper, lm_y_scale=scale)
+    assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("yper,xper", [(True, False), (False, True), (True, True)])
+def test_fft2_periodic_operator(yper, xper):
+    # Physical check with default scales: the 5-point Laplacian (periodic wrap
+    # on periodic axes) of the answer reproduces the rhs; double-periodic: the
+    # rhs with its mean removed, and the answer's mean is the constant
+    # slx*sly/4 that the zero-mode coefficient 1 adds.
+    ny, nx = (16 if yper else 15), (16 if xper else 15)
+    dx, dy = 0.2, 0.3
+    ypoints, xpoints = (ny if yper else ny + 1), (nx if xper else nx + 1)
+    lx, ly = xpoints * dx, ypoints * dy
+    rhs = O.field(22, (ny, nx))
+    u = O.lapl_rect_fft2(rhs, dx, dy, lx, ly, yperiodic=yper, xperiodic=xper)
+
+    def lap(u):
+        uy = np.roll(u, 1, 0) + np.roll(u, -1, 0) if yper else np.pad(u, ((1, 1), (0, 0)))[:-2] + np.pad(u, ((1, 1), (0, 0)))[2:]
+        ux = np.roll(u, 1, 1) + np.roll(u, -1, 1) if xper else np.pad(u, ((0, 0), (1, 1)))[:, :-2] + np.pad(u, ((0, 0), (1, 1)))[:, 2:]
+        return (uy - 2 * u) / dy ** 2 + (ux - 2 * u) / dx ** 2
+    if yper and xper:
+        assert rel(lap(u), rhs - rhs.mean()) < 1e-12
+        assert abs(u.mean() - np.sqrt(2 / lx) * np.sqrt(2 / ly) / 4) < 1e-12
+    else:
+        assert rel(lap(u), rhs) < 1e-12
+
+
 def test_rect_yperiodic_dense():
     nx, ny = 5, 8
     dx, dy = 0.3, 0.25
