@@ -177,12 +177,12 @@ def algorithmic_bytes(kernel: str, mode: str, iters: int, n: int, batch: int, fu
 PROFILE_KERNELS = [
     ("xy_fwd", "forward planes, f64 -> f32 (solve 1)", "plane_dst_kernel<double, float, float, 7, 0, 0>", 12),
     ("plane_f32", "planes f32 -> f32 (solve 1 inverse, solve 2 forward)", "plane_dst_kernel<float, float, float, 7, 0, 0>", 8),
-    ("zcr", "z cyclic reduction (FP32)", "zcr_strip_kernel<float, float, 7, 1>", 8),
+    ("zcr", "z cyclic reduction (FP32)", "zcr_strip_kernel<float, float, 7, 1, 16>", 8),
     ("residual_rhs", "FP64 residual -> FP32 rhs (solve 2)", "residual_tile_kernel<128, 8, true, float>", 16),
     ("xy_inv_acc", "inverse planes, u = u1 + c (solve 2)", "plane_dst_kernel<float, double, float, 7, 2, 0>", 16),
     ("residual", "FP64 residual, norm only", "residual_tile_kernel<128, 8, false, double>", 16),
     ("xy_f64", "planes FP64 (plain solve)", "plane_dst_kernel<double, double, double, 7, 0, 0>", 16),
-    ("zcr_f64", "z cyclic reduction (FP64)", "zcr_strip_kernel<double, double, 7, 1>", 16),
+    ("zcr_f64", "z cyclic reduction (FP64)", "zcr_strip_kernel<double, double, 7, 1, 16>", 16),
 ]
 # timing class -> profile key, where they differ
 PROFILE_KEY_OF_CLASS = {"xy_inv": "plane_f32", "xy_fwd_r32": "plane_f32"}
