@@ -30,7 +30,8 @@ OK, ERR_INVALID_SIZE, ERR_INVALID_ARG, ERR_UNSUPPORTED = 0, 1, 2, 3
 ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_ADMISSION = 4, 5, 6, 7, 8
 
 # every entry point include/fdmgpu.h declares
-EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdmgpu_rect_set_scales", "fdmgpu_set_refine",
+EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdmgpu_plan_cyl3_fft2",
+           "fdmgpu_rect_set_scales", "fdmgpu_set_refine",
            "fdmgpu_solve_host", "fdmgpu_solve_device", "fdmgpu_solve_batch_device", "fdmgpu_solve_table_device",
            "fdmgpu_plan_variants", "fdmgpu_plan_set_variant", "fdmgpu_plan_get_variant", "fdmgpu_get_stats",
            "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_group_create",
@@ -104,6 +105,7 @@ def _lib_handle():
         L.fdmgpu_plan_cube.argtypes = [C.POINTER(vp), i, i, d, d, d, d, d, d, i, i, i]
         L.fdmgpu_plan_rect.argtypes = [C.POINTER(vp), i, i, d, d, d, d, i, i]
         L.fdmgpu_plan_rect_fft2.argtypes = [C.POINTER(vp), i, i, d, d, d, d, i, i]
+        L.fdmgpu_plan_cyl3_fft2.argtypes = [C.POINTER(vp), i, i, d, d, d, d, d, i, i, i]
         L.fdmgpu_rect_set_scales.argtypes = [vp, vp, vp, vp]
         L.fdmgpu_set_refine.argtypes = [vp, d, i]
         L.fdmgpu_solve_host.argtypes = [vp, vp, vp]
@@ -379,6 +381,23 @@ class LaplRectFFT2(LaplRect):
         _check(_lib_handle().fdmgpu_plan_rect_fft2(C.byref(h), p, mask, dx, dy, lx, ly, nx, ny), "LaplRectFFT2")
         _Plan.__init__(self, h, p, (ny, nx))
         self.dx, self.dy, self.lx, self.ly, self.nx, self.ny = dx, dy, lx, ly, nx, ny
+
+
+class LaplCyl3FFT2(_Plan):
+    """LaplCyl3FFT2<T,check,zflag,true>(dr,dz,r0,lr,lz,nr,nz,nphi) (reference
+    src/lapl_cyl.h:173-249): arrays [nphi][nz][nr], r fastest; zperiodic =
+    zflag periodic (z indices 0..nz-1).  The r systems are solved by cyclic
+    reduction (the use_cyclic_reduction instantiation)."""
+    _kind = "LaplCyl3FFT2"
+
+    def __init__(self, dr, dz, r0, lr, lz, nr, nz, nphi, precision="f64", zperiodic=False):
+        h = C.c_void_p()
+        p = _prec(precision)
+        _check(_lib_handle().fdmgpu_plan_cyl3_fft2(C.byref(h), p, PERIODIC_Z if zperiodic else 0, dr, dz, r0, lr, lz,
+                                                   nr, nz, nphi), "LaplCyl3FFT2")
+        super().__init__(h, p, (nphi, nz, nr))
+        self.dr, self.dz, self.r0, self.lr, self.lz = dr, dz, r0, lr, lz
+        self.nr, self.nz, self.nphi = nr, nz, nphi
 
 
 NS_FIELDS = ("u", "v", "w", "p", "x", "F", "G", "H", "RHS")

@@ -470,13 +470,17 @@ __global__ __launch_bounds__(256) void zcrp_kernel(const BatchPtr<Ts> data, cons
 // coefficients (reference: init_Mat src/lapl_rect.cpp:44-60 + gtsv :90;
 // algorithm: cyclic_reduction_general src/cyclic_reduction.h:59-107).
 //   D_j = -2/dx^2 - lmy[k]*s_j,  L_j = Ls_j/dx^2 (j>0),  U_j = Us_j/dx^2 (j<nx-1).
+// LaplCyl3FFT2's r systems (src/lapl_cyl.cpp:55-73) are the same family with a
+// second eigenvalue: row k = i*rdiv + kz, D_j = -2/dr^2 - lm_phi[i]/r_j^2 -
+// lm_z[kz] (lmy = lm_phi, s_j = 1/r_j^2, lmz = lm_z).
 // One block per row; coefficients and rhs staged in LDS.
 // ---------------------------------------------------------------------------
 template <typename Ts, typename T>
 __global__ __launch_bounds__(256) void rowcr_kernel(const BatchPtr<Ts> data, const int nx, const int q,
                                                     const Ts* __restrict__ lmy, const Ts* __restrict__ sc,
                                                     const Ts* __restrict__ Ls, const Ts* __restrict__ Us,
-                                                    const double inv_dx2, const int* __restrict__ done) {
+                                                    const double inv_dx2, const int* __restrict__ done,
+                                                    const Ts* __restrict__ lmz, const int rdiv) {
     if (done && done[blockIdx.y]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* d = reinterpret_cast<T*>(smem_raw);
@@ -485,9 +489,10 @@ __global__ __launch_bounds__(256) void rowcr_kernel(const BatchPtr<Ts> data, con
     T* bb = f + nx;
     const int k = blockIdx.x;
     const auto row = as_global(data(blockIdx.y) + (long)k * nx);
-    const Ts lk = lmy[k];
+    const Ts lk = lmy[lmz ? k / rdiv : k];
+    const double lz = lmz ? (double)lmz[k % rdiv] : 0.0;
     for (int j = threadIdx.x; j < nx; j += blockDim.x) {
-        d[j] = (T)(-2.0 * inv_dx2 - (double)(lk * sc[j]));
+        d[j] = lmz ? (T)(-2.0 * inv_dx2 - (double)(lk * sc[j]) - lz) : (T)(-2.0 * inv_dx2 - (double)(lk * sc[j]));
         e[j] = j > 0 ? (T)(Ls[j] * inv_dx2) : T(0);
         f[j] = j < nx - 1 ? (T)(Us[j] * inv_dx2) : T(0);
         bb[j] = (T)row[j];
@@ -628,7 +633,8 @@ static hipError_t rowcr_k(const RowCrArgs& a) {
     if (e != hipSuccess) return e;
     const BatchPtr<Ts> data{(Ts*)a.data, a.data_tab, a.bs};
     hipLaunchKernelGGL((rowcr_kernel<Ts, T>), dim3(a.nrows, a.batch), dim3(256), lds, a.stream, data, a.nx, q,
-                       (const Ts*)a.lmy, (const Ts*)a.sc, (const Ts*)a.Ls, (const Ts*)a.Us, a.inv_dx2, a.done);
+                       (const Ts*)a.lmy, (const Ts*)a.sc, (const Ts*)a.Ls, (const Ts*)a.Us, a.inv_dx2, a.done,
+                       (const Ts*)a.lmz, a.rdiv);
     return hipGetLastError();
 }
 

@@ -233,6 +233,8 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
         }
         for (int tx : (P->kind == 3 || P->fft2 ? tiles(P->logx) : std::vector<int>{0}))
             for (int ty : tiles(P->logy)) {
+                if (P->kind == 4 && !(f64 ? tile_ok<double>(P->logx, ty) : tile_ok<float>(P->logx, ty)))
+                    continue;   // the phi pass shares tile_y
                 Variant v;
                 v.plane = 0;
                 v.tile_x = tx;
@@ -292,6 +294,14 @@ int harvest_timing(fdmgpu_plan_t* P) {
     return FDMGPU_OK;
 }
 
+// LaplCyl3FFT2's phi transforms use 1/sqrt(pi) held in T (SQRT_M_1_PI,
+// src/lapl_cyl.h:175): forward factor dphi * SQRT_M_1_PI, inverse SQRT_M_1_PI.
+constexpr double kSqrt1Pi = 0.56418958354775629;
+template <typename T>
+double cyl_phi_scale(const fdmgpu_plan_t* P) {
+    return (2 * M_PI / (double)P->nz) * (double)(T)kSqrt1Pi;
+}
+
 // ---- the three stages, templated on compute type T ------------------------
 // Every stage addresses problem p of a batch at ptr + p * elems, or at tab[p]
 // when a device table of per-problem pointers is given (tab != null).
@@ -302,6 +312,19 @@ int stage_fwd(fdmgpu_plan_t* P, const void* in, Tab in_tab, void* out, Tab out_t
               const int* done, int plane_class = FDMGPU_K_XY_FWD) {
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = P->dx * std::sqrt(2. / P->lx), sy = P->dy * std::sqrt(2. / P->ly);
+    if (P->kind == 4) {   // LaplCyl3FFT2: phi (slowest axis) then z, as "y" passes
+        AxisArgs ap{in, out, 0, P->nx * P->ny, 1, elems, elems, count, cyl_phi_scale<T>(P), P->tab_x, s, false, done,
+                    kRdftFwd, in_tab, out_tab, P->var.tile_y};
+        {
+            Timed tm(P, FDMGPU_K_Y_TILE, s);
+            HIPCHK((launch_yaxis<Tin, T, T>(P->logx, ap)));
+        }
+        AxisArgs az{out, out, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kfwd, out_tab,
+                    out_tab, P->var.tile_y};
+        Timed tm(P, FDMGPU_K_Y_TILE, s);
+        HIPCHK((launch_yaxis<T, T, T>(P->logy, az)));
+        return FDMGPU_OK;
+    }
     if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
         PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done, P->kfwd, in_tab, out_tab};
         Timed tm(P, plane_class, s);
@@ -344,6 +367,20 @@ int stage_inv(fdmgpu_plan_t* P, void* in, Tab in_tab, void* out, Tab out_tab, in
               const int* done, const float* base = nullptr) {
     const long elems = (long)P->nx * P->ny * P->nz;
     const double sx = std::sqrt(2. / P->lx), sy = std::sqrt(2. / P->ly);
+    if (P->kind == 4) {   // LaplCyl3FFT2: inverse z, then inverse phi
+        if (accum || base) return FDMGPU_ERR_INVALID_ARG;
+        AxisArgs az{in, in, 0, P->nx, P->nz, elems, elems, count, sy, P->tab_y, s, false, done, P->kinv, in_tab,
+                    in_tab, P->var.tile_y};
+        {
+            Timed tm(P, FDMGPU_K_Y_TILE, s);
+            HIPCHK((launch_yaxis<T, T, T>(P->logy, az)));
+        }
+        AxisArgs ap{in, out, 0, P->nx * P->ny, 1, elems, elems, count, (double)(T)kSqrt1Pi, P->tab_x, s, false, done,
+                    kRdftInv, in_tab, out_tab, P->var.tile_y};
+        Timed tm(P, FDMGPU_K_Y_TILE, s);
+        HIPCHK((launch_yaxis<T, Tout, T>(P->logx, ap)));
+        return FDMGPU_OK;
+    }
     if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
         PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv, in_tab, out_tab,
                     base};
@@ -401,6 +438,13 @@ int stage_tridiag(fdmgpu_plan_t* P, void* data, Tab data_tab, int count, hipStre
                   P->precision == FDMGPU_F32, data_tab, P->var.zcr_gmul};
         Timed tm(P, FDMGPU_K_ZCR, s);
         HIPCHK(launch_zcr<T>(P->logz, z));
+        return FDMGPU_OK;
+    }
+    if (P->kind == 4) {   // LaplCyl3FFT2: per (phi, z) mode the r system (lm_phi per row group, lm_z per row)
+        RowCrArgs r{data, P->nx, P->ny * P->nz, P->lmx, P->d_sc, P->d_Ls, P->d_Us, 1.0 / (P->dx * P->dx), elems,
+                    count, s, done, P->precision == FDMGPU_F32, data_tab, P->lmy, P->ny};
+        Timed tm(P, FDMGPU_K_ROWCR, s);
+        HIPCHK(launch_rowcr<T>(r));
         return FDMGPU_OK;
     }
     if (P->fft2) {
@@ -848,6 +892,56 @@ int fdmgpu_plan_rect_fft2(fdmgpu_plan* plan, int precision, int periodic_mask, d
     if ((rc = alloc_tables(P)) || (rc = f32 ? upload<float>(&P->lmy, lmy) : upload<double>(&P->lmy, lmy)) ||
         (rc = f32 ? upload<float>(&P->lmx, lmx) : upload<double>(&P->lmx, lmx)) ||
         (rc = set_scales_impl(P, nullptr, nullptr, nullptr))) {
+        free_plan(P);
+        return rc;
+    }
+    P->var.plane = 0;
+    apply_tuned(P);
+    *plan = P;
+    return FDMGPU_OK;
+}
+
+int fdmgpu_plan_cyl3_fft2(fdmgpu_plan* plan, int precision, int periodic_mask, double dr, double dz, double r0,
+                          double lr, double lz, int nr, int nz, int nphi) {
+    if (!plan || !valid_precision(precision)) return FDMGPU_ERR_INVALID_ARG;
+    *plan = nullptr;
+    if (periodic_mask != 0 && periodic_mask != FDMGPU_PERIODIC_Z) return FDMGPU_ERR_INVALID_ARG;
+    if (nr < 1 || nz < 1 || nphi < 1 || !(dr > 0) || !(dz > 0)) return FDMGPU_ERR_INVALID_SIZE;
+    const bool zper = periodic_mask != 0;
+    const int lgp = ilog2_exact(nphi), lgz = ilog2_exact(zper ? nz : nz + 1);
+    if (lgp < 0 || lgz < 0) return FDMGPU_ERR_INVALID_SIZE;
+    if (lgp < 1 || lgp > 9 || lgz < 1 || lgz > 9 || nr > 4096) return FDMGPU_ERR_UNSUPPORTED;
+    if (precision == FDMGPU_F32_REFINE64) return FDMGPU_ERR_UNSUPPORTED;
+    for (int j = 1; j <= nr; j++)
+        if (!(r0 + j * dr > 0)) return FDMGPU_ERR_INVALID_ARG;   // r appears as 1/r and 1/r^2
+    fdmgpu_plan_t* P = new (std::nothrow) fdmgpu_plan_t();
+    if (!P) return FDMGPU_ERR_OUT_OF_MEMORY;
+    int rc = common_init(P, precision);
+    if (rc) { free_plan(P); return rc; }
+    // array [nphi][nz][nr]: x = r (rows of the tridiagonal), y = z, z = phi
+    P->kind = 4;
+    P->periodic = periodic_mask;
+    P->kfwd = zper ? kRdftFwd : kDst;   // z axis
+    P->kinv = zper ? kRdftInv : kDst;
+    P->nx = nr; P->ny = nz; P->nz = nphi;
+    P->logx = lgp; P->logy = lgz; P->logz = 0;   // tab_x: phi tables, tab_y: z tables
+    P->dx = dr; P->dy = dz; P->lx = lr; P->ly = lz;
+    const bool as_float = precision == FDMGPU_F32;
+    const std::vector<double> lm_phi = periodic_lambda(nphi, 2 * M_PI / nphi, as_float);   // 4/dphi^2 sin^2(i dphi/2)
+    const std::vector<double> lm_z = zper ? periodic_lambda(nz, dz, as_float) : dirichlet_lambda(nz, dz, as_float);
+    std::vector<double> sc(nr), Ls(nr), Us(nr);
+    for (int j = 1; j <= nr; j++) {
+        const double r = r0 + j * dr;
+        sc[j - 1] = 1.0 / (r * r);
+        Ls[j - 1] = (r - 0.5 * dr) / r;
+        Us[j - 1] = (r + 0.5 * dr) / r;
+    }
+    const bool f32 = precision != FDMGPU_F64;
+    if ((rc = alloc_tables(P)) || (rc = f32 ? upload<float>(&P->lmx, lm_phi) : upload<double>(&P->lmx, lm_phi)) ||
+        (rc = f32 ? upload<float>(&P->lmy, lm_z) : upload<double>(&P->lmy, lm_z)) ||
+        (rc = f32 ? upload<float>(&P->d_sc, sc) : upload<double>(&P->d_sc, sc)) ||
+        (rc = f32 ? upload<float>(&P->d_Ls, Ls) : upload<double>(&P->d_Ls, Ls)) ||
+        (rc = f32 ? upload<float>(&P->d_Us, Us) : upload<double>(&P->d_Us, Us))) {
         free_plan(P);
         return rc;
     }

@@ -77,6 +77,19 @@ int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double
 int fdmgpu_plan_rect_fft2(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double lx,
                           double ly, int nx, int ny);
 
+/* Replaces LaplCyl3FFT2<T,check,zflag,use_cyclic_reduction>::LaplCyl3FFT2(dr, dz,
+ * r0, lr, lz, nr, nz, nphi) (src/lapl_cyl.h:173-249; solve
+ * src/lapl_cyl.cpp:11-128): the cylindrical Poisson solve on [nphi][nz][nr]
+ * (r fastest; z indices 1..nz, or 0..nz-1 when z is periodic) by real DFTs
+ * along phi, sine or real DFTs along z and, per (phi, z) mode, the r
+ * tridiagonal system solved by cyclic reduction (the use_cyclic_reduction
+ * instantiation; the default one factors the same systems with LAPACK
+ * gttrf/gttrs).  periodic_mask: 0 (zflag none) or FDMGPU_PERIODIC_Z.  nphi and
+ * nz+1 (nz periodic) powers of two up to 512, nr <= 4096.  FDMGPU_F32 or
+ * FDMGPU_F64 (no FP32+refine: the residual stencil is not built). */
+int fdmgpu_plan_cyl3_fft2(fdmgpu_plan* plan, int precision, int periodic_mask, double dr, double dz, double r0,
+                          double lr, double lz, int nr, int nz, int nphi);
+
 /* Replaces the public LaplRect members lm_y_scale, L_scale, U_scale
  * (src/lapl_rect.h:32, 57-59): arrays of nx+1 doubles indexed 1..nx (entry 0
  * unused, as in the reference); NULL keeps the default of all ones.  Returns
@@ -172,6 +185,7 @@ int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launc
  * No reference counterpart: the reference solves one problem per call
  * (LaplCube::solve).  A problem names its solver (the constructor arguments
  * of LaplCube or LaplRect) and its device arrays. */
+#define FDMGPU_KIND_CYL 4   /* plans only (not in groups or the server) */
 #define FDMGPU_KIND_RECT 2
 #define FDMGPU_KIND_CUBE 3
 typedef struct fdmgpu_problem {

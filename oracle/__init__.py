@@ -60,6 +60,11 @@ def lib():
         L.orc_lapl_rect_f.restype = _i
         L.orc_lapl_rect_fft2_d.argtypes = [_dp, _dp, _i, _i, _d, _d, _d, _d, _i, _i, C.c_void_p]
         L.orc_lapl_rect_fft2_d.restype = _i
+        cyl = [_i, _d, _d, _d, _d, _i, _i, _i, _i]
+        L.orc_lapl_cyl3_fft2_d.argtypes = [_dp, _dp] + cyl
+        L.orc_lapl_cyl3_fft2_f.argtypes = [_fp, _fp] + cyl
+        L.orc_lapl_cyl3_fft2_d.restype = _i
+        L.orc_lapl_cyl3_fft2_f.restype = _i
         L.orc_gtsv_d.argtypes = [_i, _dp, _dp, _dp, _dp]
         L.orc_gtsv_d.restype = _i
         L.orc_gtsv_f.argtypes = [_i, _fp, _fp, _fp, _fp]
@@ -219,6 +224,19 @@ def lapl_rect_fft2(rhs: np.ndarray, dx, dy, lx, ly, yperiodic=False, xperiodic=F
                                    _scales_ptr(lm_y_scale, np.float64, keep))
     if rc:
         raise ValueError("LaplRectFFT2: transform lengths must be powers of two")
+    return ans
+
+
+def lapl_cyl3_fft2(rhs: np.ndarray, dr, dz, r0, lz, zperiodic=False, cyclic_reduction=False) -> np.ndarray:
+    """LaplCyl3FFT2 (src/lapl_cyl.cpp:11-128): rhs [nphi][nz][nr], r fastest."""
+    nphi, nz, nr = rhs.shape
+    dt = np.float32 if rhs.dtype == np.float32 else np.float64
+    r = _c(rhs, dt)
+    ans = np.zeros_like(r)
+    f = lib().orc_lapl_cyl3_fft2_f if dt == np.float32 else lib().orc_lapl_cyl3_fft2_d
+    rc = f(ans, r, int(zperiodic), dr, dz, r0, lz, nr, nz, nphi, int(cyclic_reduction))
+    if rc:
+        raise ValueError(f"LaplCyl3FFT2: invalid sizes or singular r system (code {rc})")
     return ans
 
 

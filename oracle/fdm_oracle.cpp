@@ -727,6 +727,114 @@ inline double splitmix_value(uint64_t seed, uint64_t idx) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// LaplCyl3FFT2<T, check, zflag, use_cyclic_reduction>::solve
+// (src/lapl_cyl.cpp:11-128) with LaplCyl3Data (src/lapl_cyl.h:12-37) and
+// init_solver (src/lapl_cyl.cpp:130-170).  Arrays [nphi][nz][nr] (r fastest)
+// over the index ranges {0..nphi-1, z1..zn, 1..nr}.  The r systems: gtsv
+// (LU with partial pivoting, the gttrf/gttrs pair of the default
+// instantiation) or the general cyclic reduction (use_cyclic_reduction).
+// ---------------------------------------------------------------------------
+template <typename T>
+int lapl_cyl3_fft2_solve(T* ans, const T* rhs, int zper, double dr, double dz, double r0, double lz, int nr,
+                         int nz, int nphi, int use_cr) {
+    constexpr T SQRT_M_1_PI = (T)0.56418958354775629;
+    const double dphi = 2 * M_PI / nphi, dr2 = dr * dr, dz2 = dz * dz, dphi2 = dphi * dphi;
+    const double slz = std::sqrt(2. / lz);
+    const int zpoints = zper ? nz : nz + 1, z1 = zper ? 0 : 1, zn = zper ? nz - 1 : nz;
+    const int nrq = (int)std::ceil(std::log2(nr + 1));
+    if (!is_pow2(nphi) || !is_pow2(zpoints) || nr < 1) return -1;
+    std::vector<T> lm_phi(nphi), lm_z(zpoints);
+    for (int i = 0; i < nphi; i++) {
+        const double s = std::sin(i * dphi * 0.5);
+        lm_phi[i] = (T)(4.0 / dphi2 * (s * s));
+    }
+    for (int k = 0; k < zpoints; k++) {
+        const double s = zper ? std::sin(k * M_PI / zpoints) : std::sin(k * M_PI * 0.5 / zpoints);
+        lm_z[k] = (T)(4. / dz2 * (s * s));
+    }
+    const int zc = zn - z1 + 1;
+    auto at = [&](int i, int k, int j) { return ((size_t)i * zc + (k - z1)) * nr + (j - 1); };
+    std::vector<T> W((size_t)nphi * zc * nr);
+    const int maxp = std::max(nphi, zpoints);
+    auto phi_pass = [&](const T* src, T* dst, T factor, bool inverse) {
+        CFFT<T> f(nphi);
+#pragma omp parallel
+        {
+            std::vector<T> sa(maxp + 1), Sa(maxp + 1);
+            std::vector<std::complex<T>> buf(2 * maxp + 2);
+#pragma omp for collapse(2) schedule(static)
+            for (int k = z1; k <= zn; k++)
+                for (int j = 1; j <= nr; j++) {
+                    for (int i = 0; i < nphi; i++) sa[i] = src[at(i, k, j)];
+                    if (!inverse) rdft_fwd<T>(f, nphi, sa.data(), Sa.data(), factor, buf.data());
+                    else rdft_inv<T>(f, nphi, sa.data(), Sa.data(), factor, buf.data());
+                    for (int i = 0; i < nphi; i++) dst[at(i, k, j)] = Sa[i];
+                }
+        }
+    };
+    auto z_pass = [&](const T* src, T* dst, T factor, bool inverse) {
+        CFFT<T> f(zper ? zpoints : 2 * zpoints);
+#pragma omp parallel
+        {
+            std::vector<T> sa(maxp + 1), Sa(maxp + 1);
+            std::vector<std::complex<T>> buf(2 * maxp + 2);
+#pragma omp for collapse(2) schedule(static)
+            for (int i = 0; i < nphi; i++)
+                for (int j = 1; j <= nr; j++) {
+                    for (int k = z1; k <= zn; k++) sa[k] = src[at(i, k, j)];
+                    if (!zper) dst_pair<T>(f, zpoints, sa.data(), nullptr, Sa.data(), nullptr, factor, buf.data());
+                    else if (!inverse) rdft_fwd<T>(f, zpoints, sa.data(), Sa.data(), factor, buf.data());
+                    else rdft_inv<T>(f, zpoints, sa.data(), Sa.data(), factor, buf.data());
+                    for (int k = z1; k <= zn; k++) dst[at(i, k, j)] = Sa[k];
+                }
+        }
+    };
+    phi_pass(rhs, W.data(), (T)(dphi * SQRT_M_1_PI), false);
+    z_pass(W.data(), W.data(), (T)(dz * slz), false);
+    int info_all = 0;
+#pragma omp parallel
+    {
+        std::vector<T> L(nr), D(nr), U(nr), b(nr);
+#pragma omp for collapse(2) schedule(static)
+        for (int i = 0; i < nphi; i++)
+            for (int k = z1; k <= zn; k++) {
+                for (int j = 1; j <= nr; j++) {
+                    const double r = r0 + j * dr;
+                    D[j - 1] = (T)(-2 / dr2 - lm_phi[i] / r / r - lm_z[k]);
+                    L[j - 1] = (T)((r - 0.5 * dr) / dr2 / r);   // row j's sub-diagonal (unused at j = 1)
+                    U[j - 1] = (T)((r + 0.5 * dr) / dr2 / r);   // row j's super-diagonal (unused at j = nr)
+                    b[j - 1] = W[at(i, k, j)];
+                }
+                if (use_cr) {
+                    std::vector<T> e(nr), f(nr);
+                    for (int j = 0; j < nr; j++) {
+                        e[j] = j > 0 ? L[j] : T(0);
+                        f[j] = j < nr - 1 ? U[j] : T(0);
+                    }
+                    cyclic_reduction<T>(D.data(), e.data(), f.data(), b.data(), nrq, nr, true);
+                } else {
+                    // gtsv: dl[j] = A(j+1, j) (row j+1's sub-diagonal), du[j] = A(j, j+1)
+                    std::vector<T> dl(nr), du(nr);
+                    for (int j = 0; j + 1 < nr; j++) {
+                        dl[j] = L[j + 1];
+                        du[j] = U[j];
+                    }
+                    const int info = gtsv<T>(nr, dl.data(), D.data(), du.data(), b.data());
+                    if (info) {
+#pragma omp critical
+                        info_all = info;
+                    }
+                }
+                for (int j = 1; j <= nr; j++) W[at(i, k, j)] = b[j - 1];
+            }
+    }
+    if (info_all) return info_all;
+    z_pass(W.data(), W.data(), (T)slz, true);
+    phi_pass(W.data(), ans, SQRT_M_1_PI, true);
+    return 0;
+}
+
 extern "C" {
 
 // ----- transforms ----------------------------------------------------------
@@ -793,6 +901,17 @@ int orc_lapl_rect_f(float* ans, const float* rhs, int yper, double dx, double dy
 int orc_lapl_rect_fft2_d(double* ans, const double* rhs, int yper, int xper, double dx, double dy,
                          double lx, double ly, int nx, int ny, const double* lm_y_scale) {
     return lapl_rect_fft2_solve<double>(ans, rhs, yper, xper, dx, dy, lx, ly, nx, ny, lm_y_scale);
+}
+
+// PINNED-BY: test_cyl3_dense, test_cyl3_operator, test_cyl3_known_answer_second_order
+//            (ut/ut_lapl_cyl.cpp:320-406).
+int orc_lapl_cyl3_fft2_d(double* ans, const double* rhs, int zper, double dr, double dz, double r0, double lz,
+                         int nr, int nz, int nphi, int use_cr) {
+    return lapl_cyl3_fft2_solve<double>(ans, rhs, zper, dr, dz, r0, lz, nr, nz, nphi, use_cr);
+}
+int orc_lapl_cyl3_fft2_f(float* ans, const float* rhs, int zper, double dr, double dz, double r0, double lz,
+                         int nr, int nz, int nphi, int use_cr) {
+    return lapl_cyl3_fft2_solve<float>(ans, rhs, zper, dr, dz, r0, lz, nr, nz, nphi, use_cr);
 }
 
 // ----- tridiagonal ----------------------------------------------------------

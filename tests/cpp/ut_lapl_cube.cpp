@@ -12,6 +12,7 @@
 
 #include "fdm/lapl_cube.h"
 #include "fdm/lapl_rect.h"
+#include "fdm/lapl_cyl.h"
 
 static int failures = 0;
 #define CHECK(cond, ...)                                  \
@@ -120,6 +121,42 @@ double rect_fft1_fft2(int n) {
     return err;
 }
 
+// ut/ut_lapl_cyl.cpp:320-406: second-order convergence of LaplCyl3FFT2 on
+// u = (z-h0)(z-h1)(r-r0)(r-R)(sin phi + cos phi) (cell-centred r, z nodes;
+// rhs = the continuous Laplacian P[Z(4r-r0-R)/r + 2Rr - Rr Z/r^2], boundary
+// values moved to the rhs).  Returns max|u_h - u| / max|u|.
+double cyl_known_answer(int nr, int nz, int nphi) {
+    const double r0 = M_PI / 2, R = M_PI, h0 = 0, h1 = 10;
+    const double dr = (R - r0) / nr, dz = (h1 - h0) / nz, dphi = 2 * M_PI / nphi;
+    auto u = [&](int i, int k, int j) {
+        const double r = r0 + dr * j - dr / 2, z = h0 + dz * k - dz / 2, phi = dphi * (i + 1) - dphi / 2;
+        return (z - h0) * (z - h1) * (r - r0) * (r - R) * (std::sin(phi) + std::cos(phi));
+    };
+    fdm::LaplCyl3FFT2<double, true> lapl(dr, dz, r0 - dr / 2, R - r0 + dr, h1 - h0 + dz, nr, nz, nphi);
+    std::vector<double> rhs((size_t)nphi * nz * nr), sol(rhs.size());
+    for (int i = 0; i < nphi; i++)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= nr; j++) {
+                const double r = r0 + dr * j - dr / 2, z = h0 + dz * k - dz / 2, phi = dphi * (i + 1) - dphi / 2;
+                const double Z = (z - h0) * (z - h1), Rr = (r - r0) * (r - R), P = std::sin(phi) + std::cos(phi);
+                double f = P * (Z * (4 * r - r0 - R) / r + 2 * Rr - Rr * Z / (r * r));
+                if (j == 1) f -= (r - dr / 2) / r * u(i, k, 0) / (dr * dr);
+                if (j == nr) f -= (r + dr / 2) / r * u(i, k, nr + 1) / (dr * dr);
+                if (k == 1) f -= u(i, 0, j) / (dz * dz);
+                if (k == nz) f -= u(i, nz + 1, j) / (dz * dz);
+                rhs[((size_t)i * nz + (k - 1)) * nr + (j - 1)] = f;
+            }
+    lapl.solve(sol.data(), rhs.data());
+    double err = 0, nrm = 0;
+    for (int i = 0; i < nphi; i++)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= nr; j++) {
+                err = std::max(err, std::fabs(sol[((size_t)i * nz + (k - 1)) * nr + (j - 1)] - u(i, k, j)));
+                nrm = std::max(nrm, std::fabs(u(i, k, j)));
+            }
+    return err / nrm;
+}
+
 // periodic^3 on [0, 2pi)^3, n = 32: u = sin^3 x + cos^5 y - sin z with its
 // analytic Laplacian as rhs; the solve's mean is removed and the max error
 // must stay below 1e-2 (ut/ut_lapl_cube.cpp:134-244).
@@ -178,6 +215,12 @@ int main() {
     std::printf("LaplRect<float> 31^2 biquadratic: err %.3e\n", e);
     CHECK(e < 1e-5, "rect float err %g", e);
 
+    {
+        const double e1 = cyl_known_answer(16, 15, 16), e2 = cyl_known_answer(32, 31, 32);
+        std::printf("LaplCyl3FFT2<double> known answer: err %.3e (16,15,16), %.3e (32,31,32), ratio %.2f\n", e1, e2,
+                    e1 / e2);
+        CHECK(e1 / e2 > 3.7, "cyl convergence ratio %g", e1 / e2);
+    }
     e = rect_fft1_fft2(511);
     std::printf("LaplRect vs LaplRectFFT2<double> 511^2: max abs diff %.3e\n", e);
     CHECK(e < 1e-13, "rect fft1/fft2 diff %g", e);

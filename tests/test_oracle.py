@@ -440,6 +440,94 @@ def test_fft2_periodic_operator(yper, xper):
         assert rel(lap(u), rhs) < 1e-12
 
 
+def _cyl_geometry(nr, nz, nphi, zper):
+    dr, dz, r0 = 0.11, 0.2, 0.35
+    zpoints = nz if zper else nz + 1
+    return dr, dz, r0, zpoints * dz
+
+
+def _cyl_apply(u, dr, dz, r0, zper):
+    """The discrete operator LaplCyl3FFT2 inverts (src/lapl_cyl.cpp:55-73,
+    130-170): flux-form r second difference, periodic phi second difference
+    over r^2, z second difference (periodic wrap or zero Dirichlet)."""
+    nphi, nz, nr = u.shape
+    dphi = 2 * np.pi / nphi
+    r = r0 + dr * np.arange(1, nr + 1)
+    up = np.pad(u, ((0, 0), (0, 0), (1, 1)))
+    Ar = ((r + 0.5 * dr) * (up[:, :, 2:] - u) - (r - 0.5 * dr) * (u - up[:, :, :-2])) / (dr ** 2 * r)
+    Aphi = (np.roll(u, 1, 0) - 2 * u + np.roll(u, -1, 0)) / (dphi ** 2 * r ** 2)
+    if zper:
+        Az = (np.roll(u, 1, 1) - 2 * u + np.roll(u, -1, 1)) / dz ** 2
+    else:
+        uz = np.pad(u, ((0, 0), (1, 1), (0, 0)))
+        Az = (uz[:, 2:] - 2 * u + uz[:, :-2]) / dz ** 2
+    return Ar + Aphi + Az
+
+
+@pytest.mark.parametrize("nphi,nz,nr,zper", [(8, 7, 5, False), (16, 15, 9, False), (8, 8, 6, True), (32, 16, 12, True)])
+def test_cyl3_operator(nphi, nz, nr, zper):
+    # the cylindrical 7-point operator of the answer reproduces the rhs, for
+    # the LU (gttrs) and cyclic-reduction instantiations, double and float
+    dr, dz, r0, lz = _cyl_geometry(nr, nz, nphi, zper)
+    rhs = O.field(41, (nphi, nz, nr))
+    for cr in (False, True):
+        u = O.lapl_cyl3_fft2(rhs, dr, dz, r0, lz, zperiodic=zper, cyclic_reduction=cr)
+        assert rel(_cyl_apply(u, dr, dz, r0, zper), rhs) < 1e-12
+        uf = O.lapl_cyl3_fft2(rhs.astype(np.float32), dr, dz, r0, lz, zperiodic=zper, cyclic_reduction=cr)
+        assert rel(uf, u) < 2e-5
+
+
+def cyl_known_answer(nr, nz, nphi, solve):
+    """ut/ut_lapl_cyl.cpp:320-406: u = (z-h0)(z-h1)(r-r0)(r-R)(sin phi + cos phi)
+    on cell-centred r, z nodes, rhs = its continuous Laplacian
+    P[Z (4r - r0 - R)/r + 2 Rr - Rr Z / r^2] with the boundary values moved to
+    the rhs; returns max|u_h - u| / max|u|."""
+    r0, R, h0, h1 = np.pi / 2, np.pi, 0.0, 10.0
+    dr, dz, dphi = (R - r0) / nr, (h1 - h0) / nz, 2 * np.pi / nphi
+    i = np.arange(nphi)[:, None, None]
+    k = np.arange(0, nz + 2)[None, :, None]
+    j = np.arange(0, nr + 2)[None, None, :]
+    r, z, phi = r0 + dr * j - dr / 2, h0 + dz * k - dz / 2, dphi * (i + 1) - dphi / 2
+    Z, Rr, P = (z - h0) * (z - h1), (r - r0) * (r - R), np.sin(phi) + np.cos(phi)
+    u = Z * Rr * P
+    f = P * (Z * (4 * r - r0 - R) / r + 2 * Rr - Rr * Z / r ** 2)
+    rhs = f[:, 1:nz + 1, 1:nr + 1].copy()
+    rr = r[:, :, 1:nr + 1]
+    rhs[:, :, 0] -= ((rr - dr / 2) / rr)[:, :, 0] * u[:, 1:nz + 1, 0] / dr ** 2
+    rhs[:, :, -1] -= ((rr + dr / 2) / rr)[:, :, -1] * u[:, 1:nz + 1, nr + 1] / dr ** 2
+    rhs[:, 0, :] -= u[:, 0, 1:nr + 1] / dz ** 2
+    rhs[:, -1, :] -= u[:, nz + 1, 1:nr + 1] / dz ** 2
+    exact = u[:, 1:nz + 1, 1:nr + 1]
+    got = solve(rhs, dr, dz, r0 - dr / 2, R - r0 + dr, h1 - h0 + dz, nr, nz, nphi)
+    return np.abs(got - exact).max() / np.abs(exact).max()
+
+
+def test_cyl3_known_answer_second_order():
+    # the reference asserts err(16,15,16) / err(32,31,32) > 3.7
+    def solve(rhs, dr, dz, r0, lr, lz, nr, nz, nphi):
+        return O.lapl_cyl3_fft2(rhs, dr, dz, r0, lz)
+    e1 = cyl_known_answer(16, 15, 16, solve)
+    e2 = cyl_known_answer(32, 31, 32, solve)
+    assert e1 / e2 > 3.7
+    assert e2 < 1e-2
+
+
+def test_cyl3_dense():
+    # brute force: the dense operator solved directly (nonsingular: r > 0)
+    nphi, nz, nr = 8, 3, 4
+    for zper, nzz in ((False, 3), (True, 4)):
+        dr, dz, r0, lz = _cyl_geometry(nr, nzz, nphi, zper)
+        n = nphi * nzz * nr
+        A = np.zeros((n, n))
+        for c in range(n):
+            e = np.zeros(n)
+            e[c] = 1
+            A[:, c] = _cyl_apply(e.reshape(nphi, nzz, nr), dr, dz, r0, zper).ravel()
+        rhs = O.field(42, (nphi, nzz, nr))
+        want = np.linalg.solve(A, rhs.ravel()).reshape(rhs.shape)
+        assert rel(O.lapl_cyl3_fft2(rhs, dr, dz, r0, lz, zperiodic=zper), want) < 1e-12
+
+
 def test_rect_yperiodic_dense():
     nx, ny = 5, 8
     dx, dy = 0.3, 0.25
