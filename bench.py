@@ -21,6 +21,9 @@ Legs (each can be skipped):
   c5    NSCube lid-driven cavity 127^3 (Re 250, dt 0.01, u0 1), 1000 steps
         with the FP32+refine projection; steps/s; compared with the CPU
         oracle's FP64 NSCube at step 100.
+  pub   periodic LaplCube 32^3 / 64^3 / 128^3 solved one at a time (FP64
+        and FP32) beside the reference's published n-body Poisson-stage
+        timings (test/nbody_result.org).
   cpu   the reference's LaplCube<double> on the host cores (bounded sample).
 
 Prints ONE JSON line.
@@ -542,6 +545,53 @@ def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 32
     return out
 
 
+# The reference's only published timings for this path: its n-body code's
+# "Poisson" stage, one periodic LaplCube solve (+ one rhs scaling sweep), in ms
+# (test/nbody_result.org:9,24,39): Apple M1 CPU / M1 GPU (Vulkan) / Ryzen 7
+# 3700X (16 OpenMP threads) / RTX 3060 (Vulkan).
+PUBLISHED_POISSON_MS = {32: (1.7, 1.21, 1.26, 0.25), 64: (8.87, 5.69, 6.90, 0.86), 128: (68.83, 40.37, 62.50, 5.50)}
+
+
+def leg_published(reps: int = 50):
+    """Periodic LaplCube N^3 (N = 32, 64, 128: all N points per axis), one
+    solve at a time on a stream, device-resident, FP64 and FP32: the
+    reference's published Poisson-stage configuration.  The reference's timed
+    window also holds one rhs scaling sweep, not timed here."""
+    import fdm_amd
+    from fdm_amd import hip
+    out = {"workload": "periodic LaplCube N^3 (tensor_flags<periodic,periodic,periodic>), one solve at a time, "
+                       "inputs resident in HBM; published: the reference's n-body Poisson stage",
+           "published_ms": {str(k): dict(zip(("m1_cpu", "m1_gpu_vulkan", "ryzen_3700x_16t", "rtx3060_vulkan"), v))
+                            for k, v in PUBLISHED_POISSON_MS.items()}}
+    st = hip.Stream()
+    for n in (32, 64, 128):
+        h = 2 * math.pi / n
+        rhs = np.random.default_rng(n).uniform(-1, 1, (n, n, n))
+        rhs -= rhs.mean()
+        row = {}
+        for prec, dt in (("f64", np.float64), ("f32", np.float32)):
+            s = fdm_amd.LaplCube(h, h, h, 2 * math.pi, 2 * math.pi, 2 * math.pi, n, n, n, precision=prec,
+                                 periodic=True)
+            d_f = hip.DeviceArray.from_host(rhs.astype(dt))
+            d_u = hip.DeviceArray(rhs.shape, dt)
+            for _ in range(3):
+                s.solve(d_u, d_f, st)
+            st.synchronize()
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(st)
+            for _ in range(reps):
+                s.solve(d_u, d_f, st)
+            e1.record(st)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / reps
+            row[prec] = {"ms_per_solve": round(ms, 4), "solves_per_s": round(1e3 / ms, 1)}
+            del s
+        row["speedup_vs_rtx3060_f64"] = round(PUBLISHED_POISSON_MS[n][3] / row["f64"]["ms_per_solve"], 1)
+        row["speedup_vs_ryzen_f64"] = round(PUBLISHED_POISSON_MS[n][2] / row["f64"]["ms_per_solve"], 1)
+        out[str(n)] = row
+    return out
+
+
 def leg_c5(n: int = 127, steps: int = 1000, check_at: int = 100, cpu_check: bool = True):
     """NSCube lid-driven cavity n^3 on [-pi,pi]^3, Re 250, dt 0.01, u0 1, from
     rest; pressure projection = the FP32+refine solve.  Compared with the CPU
@@ -640,8 +690,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mode", choices=("both", "refine", "f64"), default="both",
                     help="which GPU legs to run (profiling runs use one)")
-    ap.add_argument("--legs", default="head,c2,c3,c4,c5,cpu",
-                    help="comma list of legs: head (the value; always run), c2, c3, c4, c5, cpu")
+    ap.add_argument("--legs", default="head,c2,c3,c4,c5,pub,cpu",
+                    help="comma list of legs: head (the value; always run), c2, c3, c4, c5, pub, cpu")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving rate")
     ap.add_argument("--profile-summary", nargs=2, metavar=("KT_DIR", "PMC_DIR"),
                     help="summarise rocprofv3 csv output into profiles/ and exit (no GPU)")
@@ -752,6 +802,9 @@ def main():
         out["serve"] = leg_c4(duration_s=args.serve_seconds)
         out["p50_ms"] = [r["p50_ms"] for r in out["serve"]["rates"]]
         out["p99_ms"] = [r["p99_ms"] for r in out["serve"]["rates"]]
+    if "pub" in legs:
+        progress("pub: periodic LaplCube 32^3..128^3 one at a time (the reference's published Poisson stage)")
+        out["published_poisson"] = leg_published()
     if "c5" in legs:
         progress("c5: NSCube 127^3 cavity, 1000 steps (CPU oracle check at step 100)")
         out["nscube"] = leg_c5(cpu_check="cpu" in legs)
