@@ -57,6 +57,32 @@ __device__ __forceinline__ void plane_sweep(const int tid, F&& f) {
     }
 }
 
+// The same sweep as a two-stage copy: every global load of the thread is
+// issued into registers before the first store into LDS.  Left to itself the
+// scheduler interleaves load, wait and store per row when the transform's
+// register demand is high (FP64, N = 128), which puts one memory round trip
+// per row on the critical path.
+template <int LOGN, int NT, int NN, typename V, typename LD, typename ST>
+__device__ __forceinline__ void plane_stage(const int tid, LD&& ld, ST&& st) {
+    constexpr int N = 1 << LOGN, R = NT / N, ITER = (NN + R - 1) / R;
+    static_assert(NT % N == 0, "a block covers whole rows");
+    const int x = tid & (N - 1), r0 = tid >> LOGN;
+    if (x < NN) {
+        V v[ITER];
+#pragma unroll
+        for (int i = 0; i < ITER; ++i) {
+            const int y = r0 + R * i;
+            if (R * i + R <= NN || y < NN) v[i] = ld(y, x);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < ITER; ++i) {
+            const int y = r0 + R * i;
+            if (R * i + R <= NN || y < NN) st(y, x, v[i]);
+        }
+    }
+}
+
 // KIND (dst_device.hpp): kDst for Dirichlet planes of (N-1)^2 points,
 // kRdftFwd / kRdftInv for periodic planes of N^2 points.
 // ACC: 0 out = result; 1 out += result; 2 out = base + result with an FP32
@@ -93,7 +119,9 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
             lds[LX::at(i, 0)] = T(0);   // dummy column 0
         }
     }
-    plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) { lds[LX::at(y + o, x + o)] = (T)src[y * n + x]; });
+    plane_stage<LOGN, NT, n, Tin>(
+        tid, [&](int y, int x) { return src[y * n + x]; },
+        [&](int y, int x, Tin v) { lds[LX::at(y + o, x + o)] = (T)v; });
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, N / 2, LX, NT>(lds, L, sx, tid);   // pencils = rows    (x)
     __syncthreads();
