@@ -152,7 +152,10 @@ __global__ __launch_bounds__(256) void zcr_kernel(const BatchPtr<Ts> data, const
 #pragma unroll
     for (int r = 0; r < C; ++r) {
         const int row = c * C + r;
-        v[r] = (valid && row < NQ - 1) ? (T)base[(long)row * plane] * gamma : T(0);
+        // branch-free: the row is clamped into the field and the value
+        // selected, so all C loads are in flight together
+        const T x = (T)base[(long)(row < NQ - 1 ? row : NQ - 2) * plane];
+        v[r] = (valid && row < NQ - 1) ? x * gamma : T(0);
     }
     CrLevels<T, Q> L;
     cr_levels<T, Q>(L, b, d0 - (T)lmx[kx] - (T)lmy[ky]);
@@ -213,7 +216,10 @@ __global__ __launch_bounds__((64 * ZcrStripCfg<Q, CR>::G * S)) void zcr_strip_ke
 #pragma unroll
     for (int r = 0; r < C; ++r) {
         const int row = c * C + r;
-        v[r] = (valid && row < NQ - 1) ? (T)base[(long)row * plane] * gamma : T(0);
+        // branch-free: the row is clamped into the field and the value
+        // selected, so all C loads are in flight together
+        const T x = (T)base[(long)(row < NQ - 1 ? row : NQ - 2) * plane];
+        v[r] = (valid && row < NQ - 1) ? x * gamma : T(0);
     }
     CrLevels<T, Q> L;
     cr_levels<T, Q>(L, b, d0 - (T)lmx[kx] - (T)lmy[ky]);
@@ -368,7 +374,10 @@ __global__ __launch_bounds__(256) void zcrp_kernel(const BatchPtr<Ts> data, cons
 
     T v[C];
 #pragma unroll
-    for (int r = 0; r < C; ++r) v[r] = valid ? (T)base[(long)(c * C + r) * plane] * gamma : T(0);
+    for (int r = 0; r < C; ++r) {
+        const T x = (T)base[(long)(c * C + r) * plane];   // in the field for every lane (mm is clamped)
+        v[r] = valid ? x * gamma : T(0);
+    }
 
     if (mm != 0) {
         CrLevelsP<T, Q> L;
