@@ -355,6 +355,11 @@ def leg_c2(reps: int = 5):
     res = {}
     for mode in ("f64", "refine"):
         s = fdm_amd.LaplCube(*g, n, n, n, precision=mode)
+        if mode == "refine":
+            # two inner solves reach 1e-10 here (the result reports the residual
+            # and whether it converged); each launched but unneeded iteration
+            # costs seven no-op kernels
+            s.set_refine(1e-10, 2)
         d_u = hip.DeviceArray(f.shape, np.float64)
         s.solve(d_u, d_f, st)
         st.synchronize()
@@ -376,6 +381,8 @@ def leg_c2(reps: int = 5):
                      "kernel_ms_per_solve": {k: round(v[0] / reps, 4) for k, v in s.timing().items()}}
         if mode == "refine":
             out[mode]["rel_residual"] = stats.rel_residual
+            out[mode]["max_iter"] = 2
+            out[mode]["converged"] = bool(stats.rel_residual <= 1e-10)
         del s
     out["err_refine_vs_fp64_gpu"] = float(np.linalg.norm(res["refine"] - res["f64"]) / np.linalg.norm(res["f64"]))
     out["refine_over_fp64"] = round(out["refine"]["solves_per_s"] / out["f64"]["solves_per_s"], 3)

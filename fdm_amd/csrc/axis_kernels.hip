@@ -16,20 +16,48 @@ namespace fdmgpu {
 // ---------------------------------------------------------------------------
 // axis-tile kernels (planes that do not fit LDS, or non-square planes)
 // ---------------------------------------------------------------------------
-// TILE = pencils per block (16, 32 or 64; a tunable, see tile_ok()).
+// TILE = pencils per block (16, 32 or 64; a tunable, see tile_ok()).  Both
+// kernels hold their tile in LDS as TILE pencils of pitch P = N + N1 (the
+// pencils are rows, the layout of the plane kernels' x pass), so the
+// transform's LDS accesses are the same for x and y tiles.
 template <int LOGN, typename T, int TILE_>
 struct TileCfg {
     using S = DstShape<LOGN>;
     static constexpr int N = S::N;
     static constexpr int TILE = TILE_;
-    static constexpr int SWN1 = (TILE / 2 > RowSwizzle<S::N1>::FMAX) ? S::N1 : 1;   // swizzle only if it fits
-    static constexpr int P = N + S::PAD;            // x tiles: pencil pitch
-    static constexpr int SJ = TILE + S::N1;         // y tiles: slot pitch
+    static constexpr int P = N + S::PAD;   // pencil pitch
     static constexpr int WORK = (TILE / 2) * S::N1;
     static constexpr int NT = WORK < 64 ? 64 : WORK;
-    static constexpr size_t lds_x() { return sizeof(T) * (size_t)TILE * P; }
-    static constexpr size_t lds_y() { return sizeof(T) * (size_t)N * SJ; }
+    static constexpr size_t lds() { return sizeof(T) * (size_t)TILE * P; }
 };
+
+constexpr size_t tile_lds_bytes(int logn, int tile, size_t elem) {
+    return elem * (size_t)tile * (((size_t)1 << logn) + ((size_t)1 << (logn / 2)));
+}
+
+// Global -> LDS copy of ITEMS items over NT threads (thread tid takes items
+// tid, tid + NT, ...), in batches of KB: every load of a batch is issued into
+// registers before the batch's first LDS store, so up to KB loads per thread
+// are in flight at once.  ld(idx) is called for idx < ITER * NT and must stay
+// in bounds for all of them (the callers clamp the address and select zero);
+// st(idx, v) is called for idx < ITEMS only.
+template <int ITEMS, int NT, int KB, typename V, typename LD, typename ST>
+__device__ __forceinline__ void staged_copy(const int tid, LD&& ld, ST&& st) {
+    constexpr int ITER = (ITEMS + NT - 1) / NT;
+#pragma unroll
+    for (int b0 = 0; b0 < ITER; b0 += KB) {
+        V v[KB];
+#pragma unroll
+        for (int i = 0; i < KB; ++i)
+            if (b0 + i < ITER) v[i] = ld(tid + NT * (b0 + i));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < KB; ++i) {
+            const int idx = tid + NT * (b0 + i);
+            if (b0 + i < ITER && (ITEMS % NT == 0 || idx < ITEMS)) st(idx, v[i]);
+        }
+    }
+}
 
 // rows = nrows pencils of IO::n points (N-1 Dirichlet, N periodic), contiguous;
 // batch via blockIdx.y.
@@ -55,10 +83,17 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void xaxis_dst_kerne
     L.init(tables, tid & (S::N1 - 1));
     if constexpr (KIND == kDst)
         for (int p = tid; p < TILE; p += NT) lds[LX::at(p, 0)] = T(0);
-    for (int idx = tid; idx < TILE * n; idx += NT) {
-        const int p = idx / n, x = idx - p * n;
-        lds[LX::at(p, x + o)] = p < rows_here ? (T)src[idx] : T(0);
-    }
+    staged_copy<TILE * n, NT, 32, Tin>(
+        tid,
+        [&](int idx) {
+            const int p = idx / n, x = idx - p * n;
+            const Tin raw = src[(p < rows_here ? p : rows_here - 1) * n + x];
+            return p < rows_here ? raw : Tin(0);
+        },
+        [&](int idx, Tin v) {
+            const int p = idx / n, x = idx - p * n;
+            lds[LX::at(p, x + o)] = (T)v;
+        });
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, TILE / 2, LX, NT>(lds, L, scale, tid);
     __syncthreads();
@@ -70,7 +105,9 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void xaxis_dst_kerne
     }
 }
 
-// Columns of plane blockIdx.y (nplanes planes of [IO::n][nx]); batch via blockIdx.z.
+// Columns of plane blockIdx.y (nplanes planes of [IO::n][nx]); batch via
+// blockIdx.z.  Column p of the tile is stored as LDS row p: the global reads
+// stay row segments of TILE consecutive x, and the transform is the x pass.
 template <typename Tin, typename Tout, typename T, int LOGN, int KIND, int TILE_>
 __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void yaxis_dst_kernel(
     const BatchPtr<const Tin> in, const BatchPtr<Tout> out, const int nx, const T scale,
@@ -78,7 +115,7 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void yaxis_dst_kerne
     using C = TileCfg<LOGN, T, TILE_>;
     using S = DstShape<LOGN>;
     using IO = PencilIO<KIND, LOGN>;
-    using LY = PlaneLayout<C::SJ, C::SWN1, false>;   // rows = slots, columns = pencils
+    using LT = PlaneLayout<C::P, S::N1, true>;   // rows = pencils (columns of the plane)
     constexpr int n = IO::n, o = IO::off, NT = C::NT, TILE = C::TILE;
     if (done && done[blockIdx.z]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -92,34 +129,39 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void yaxis_dst_kerne
     DstLane<T, LOGN> L;
     L.init(tables, tid & (S::N1 - 1));
     if constexpr (KIND == kDst)
-        for (int p = tid; p < TILE; p += NT) lds[LY::addr(p, 0)] = T(0);   // slot 0 of every pencil
+        for (int p = tid; p < TILE; p += NT) lds[LT::at(p, 0)] = T(0);   // slot 0 of every pencil
+    staged_copy<n * TILE, NT, 32, Tin>(
+        tid,
+        [&](int idx) {
+            const int y = idx / TILE, x = x0 + (idx - y * TILE);
+            const Tin raw = src[(long)(y < n ? y : n - 1) * nx + (x < nx ? x : nx - 1)];
+            return x < nx ? raw : Tin(0);
+        },
+        [&](int idx, Tin v) {
+            const int y = idx / TILE, p = idx - y * TILE;
+            lds[LT::at(p, y + o)] = (T)v;
+        });
+    __syncthreads();
+    pencil_pairs<KIND, T, LOGN, TILE / 2, LT, NT>(lds, L, scale, tid);
+    __syncthreads();
     for (int idx = tid; idx < n * TILE; idx += NT) {
         const int y = idx / TILE, p = idx - y * TILE;
         const int x = x0 + p;
-        lds[LY::addr(p, y + o)] = x < nx ? (T)src[(long)y * nx + x] : T(0);
-    }
-    __syncthreads();
-    pencil_pairs<KIND, T, LOGN, TILE / 2, LY, NT>(lds, L, scale, tid);
-    __syncthreads();
-    for (int idx = tid; idx < n * TILE; idx += NT) {
-        const int y = idx / TILE, p = idx - y * TILE;
-        const int x = x0 + p;
-        if (x < nx) dst[(long)y * nx + x] = (Tout)lds[LY::addr(p, IO::out(y))];
+        if (x < nx) dst[(long)y * nx + x] = (Tout)lds[LT::at(p, IO::out(y))];
     }
 }
 
 
 // Tile sizes an axis kernel of length 2^logn supports: 16 always; 32 and 64
-// for N >= 32 when the y tile (N x (TILE + N1) elements) fits LDS.
+// for N >= 32 when the tile (TILE x (N + N1) elements) fits LDS.
 template <typename T>
 bool tile_ok(int logn, int tile) {
-    const size_t N = (size_t)1 << logn, N1 = (size_t)1 << (logn / 2);
     if (tile == 16) return true;
     if (tile != 32 && tile != 64) return false;
-    return logn >= 5 && N * (tile + N1) * sizeof(T) <= kLdsBytes;
+    return logn >= 5 && tile_lds_bytes(logn, tile, sizeof(T)) <= kLdsBytes;
 }
 
-// 32 pencils per block, or 16 when a 32-wide y tile would not fit LDS
+// 32 pencils per block, or 16 when a 32-pencil tile would not fit LDS
 template <typename T>
 int default_tile(int logn) {
     return tile_ok<T>(logn, 32) ? 32 : 16;
@@ -129,7 +171,7 @@ template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIN
 static hipError_t xaxis_k(const AxisArgs& a) {
     using C = TileCfg<LOGN, T, TILE>;
     auto k = xaxis_dst_kernel<Tin, Tout, T, LOGN, ACCUM, KIND, TILE>;
-    const size_t lds = C::lds_x();
+    const size_t lds = C::lds();
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     const long tiles = (a.nrows + C::TILE - 1) / C::TILE;
@@ -159,8 +201,9 @@ static hipError_t xaxis_t(const AxisArgs& a) {
     if (!tile_ok<T>(LOGN, tile)) return hipErrorInvalidValue;
     if (tile == 16) return xaxis_tile<Tin, Tout, T, LOGN, 16>(a);
     if constexpr (LOGN >= 5) {
-        if (tile == 32) return xaxis_tile<Tin, Tout, T, LOGN, 32>(a);
-        if constexpr ((size_t)(1 << LOGN) * (64 + (1 << (LOGN / 2))) * sizeof(T) <= kLdsBytes)
+        if constexpr (tile_lds_bytes(LOGN, 32, sizeof(T)) <= kLdsBytes)
+            if (tile == 32) return xaxis_tile<Tin, Tout, T, LOGN, 32>(a);
+        if constexpr (tile_lds_bytes(LOGN, 64, sizeof(T)) <= kLdsBytes)
             if (tile == 64) return xaxis_tile<Tin, Tout, T, LOGN, 64>(a);
     }
     return hipErrorInvalidValue;
@@ -170,7 +213,7 @@ template <typename Tin, typename Tout, typename T, int LOGN, int KIND, int TILE>
 static hipError_t yaxis_k(const AxisArgs& a) {
     using C = TileCfg<LOGN, T, TILE>;
     auto k = yaxis_dst_kernel<Tin, Tout, T, LOGN, KIND, TILE>;
-    const size_t lds = C::lds_y();
+    const size_t lds = C::lds();
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     const int tiles = (a.nx + C::TILE - 1) / C::TILE;
@@ -197,8 +240,9 @@ static hipError_t yaxis_t(const AxisArgs& a) {
     if (!tile_ok<T>(LOGN, tile)) return hipErrorInvalidValue;
     if (tile == 16) return yaxis_tile<Tin, Tout, T, LOGN, 16>(a);
     if constexpr (LOGN >= 5) {
-        if (tile == 32) return yaxis_tile<Tin, Tout, T, LOGN, 32>(a);
-        if constexpr ((size_t)(1 << LOGN) * (64 + (1 << (LOGN / 2))) * sizeof(T) <= kLdsBytes)
+        if constexpr (tile_lds_bytes(LOGN, 32, sizeof(T)) <= kLdsBytes)
+            if (tile == 32) return yaxis_tile<Tin, Tout, T, LOGN, 32>(a);
+        if constexpr (tile_lds_bytes(LOGN, 64, sizeof(T)) <= kLdsBytes)
             if (tile == 64) return yaxis_tile<Tin, Tout, T, LOGN, 64>(a);
     }
     return hipErrorInvalidValue;

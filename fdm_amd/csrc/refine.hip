@@ -251,22 +251,56 @@ static void launch_residual_tile(const ResidualArgs& a, const BatchPtr<const dou
                            a.partial, a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.batch, cg, a.done);
 }
 
-// One block per problem: fixed-order sum of the partials, then the flag.
-__global__ __launch_bounds__(kResBlock) void check_kernel(const double* __restrict__ partial, const int nblk,
-                                                         const double tol, int* __restrict__ done,
-                                                         int* __restrict__ iters, double* __restrict__ resid,
-                                                         const int it) {
-    __shared__ double sh[2 * kResBlock];
+// One block per problem: fixed-order sum of the partials, then the flag.  A
+// thread adds partials tid, tid + kCheckBlock, ... with kCheckUnroll loads in
+// flight (a 255^3 problem has 32768 partials), then the waves' sums are added
+// in wave order; the order depends on nblk only, not on the batch.
+constexpr int kCheckBlock = 1024;
+constexpr int kCheckUnroll = 8;
+
+__global__ __launch_bounds__(kCheckBlock) void check_kernel(const double* __restrict__ partial, const int nblk,
+                                                           const double tol, int* __restrict__ done,
+                                                           int* __restrict__ iters, double* __restrict__ resid,
+                                                           const int it) {
+    __shared__ double sh[2 * (kCheckBlock / 64)];
     const int p = blockIdx.x;
     if (done[p]) return;
+    const auto part = as_global(partial + (long)p * nblk * 2);
     double rr = 0.0, ff = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += kResBlock) {
-        rr += partial[((long)p * nblk + i) * 2 + 0];
-        ff += partial[((long)p * nblk + i) * 2 + 1];
+    for (int b = 0; b < nblk; b += kCheckBlock * kCheckUnroll) {
+        double vr[kCheckUnroll], vf[kCheckUnroll];
+#pragma unroll
+        for (int k = 0; k < kCheckUnroll; ++k) {
+            const int i = b + k * kCheckBlock + (int)threadIdx.x;
+            const int ic = i < nblk ? i : nblk - 1;   // clamped: no branch around the loads
+            const double x = part[2 * ic], y = part[2 * ic + 1];
+            vr[k] = i < nblk ? x : 0.0;
+            vf[k] = i < nblk ? y : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kCheckUnroll; ++k) {
+            rr += vr[k];
+            ff += vf[k];
+        }
     }
-    block_sum2<double>(rr, ff, sh);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        rr += __shfl_xor(rr, d, 64);
+        ff += __shfl_xor(ff, d, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[2 * w] = rr;
+        sh[2 * w + 1] = ff;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const double rel = ff > 0.0 ? sqrt(rr / ff) : sqrt(rr);
+        double a = 0.0, c = 0.0;
+        for (int i = 0; i < kCheckBlock / 64; ++i) {
+            a += sh[2 * i];
+            c += sh[2 * i + 1];
+        }
+        const double rel = c > 0.0 ? sqrt(a / c) : sqrt(a);
         resid[p] = rel;
         iters[p] = it;
         if (rel <= tol) done[p] = 1;
@@ -310,14 +344,14 @@ hipError_t launch_residual(const ResidualArgs& a) {
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(check_kernel, dim3(a.batch), dim3(kResBlock), 0, a.stream, a.partial, (int)nblk, a.tol, a.done,
+    hipLaunchKernelGGL(check_kernel, dim3(a.batch), dim3(kCheckBlock), 0, a.stream, a.partial, (int)nblk, a.tol, a.done,
                        a.iters, a.resid, a.it);
     return hipGetLastError();
 }
 
 hipError_t launch_check(const double* partial, int nblk, double tol, int* done, int* iters, double* resid, int it,
                         int batch, hipStream_t s) {
-    hipLaunchKernelGGL(check_kernel, dim3(batch), dim3(kResBlock), 0, s, partial, nblk, tol, done, iters, resid, it);
+    hipLaunchKernelGGL(check_kernel, dim3(batch), dim3(kCheckBlock), 0, s, partial, nblk, tol, done, iters, resid, it);
     return hipGetLastError();
 }
 
