@@ -126,7 +126,7 @@ static ResTile res_tile(int nx) {
 }
 
 template <int TX, int TY, bool WRITE, typename Tu>
-__global__ __launch_bounds__(TX* TY) void residual_tile_kernel(
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(8))) void residual_tile_kernel(
     const BatchPtr<const double> f, const BatchPtr<const Tu> u, float* __restrict__ r32,
     double* __restrict__ partial, const int nx, const int ny, const int nz, const double ix, const double iy,
     const double iz, const long bs, const int batch, const int chunk_gran, const int* __restrict__ done) {
@@ -153,44 +153,65 @@ __global__ __launch_bounds__(TX* TY) void residual_tile_kernel(
     const long pl = (long)nx * ny;
     const bool valid = x < nx && y < ny;
     const long c0 = valid ? (long)y * nx + x : 0;
-    // halo loads: row y0-1 by ty == 0, row y0+TY by ty == TY-1, columns x0-1 / x0+TX by tx == 0 / TX-1
-    const bool hN = ty == 0 && y > 0 && x < nx;
-    const bool hS = ty == TY - 1 && y + 1 < ny && x < nx;
-    const int xh = tx == 0 ? x - 1 : x + 1;
-    const bool hW = (tx == 0 || tx == TX - 1) && y < ny && xh >= 0 && xh < nx;
-    const long cN = hN ? (long)(y - 1) * nx + x : 0, cS = hS ? (long)(y + 1) * nx + x : 0;
-    const long cW = hW ? (long)y * nx + xh : 0;
-    const int wcol = tx == 0 ? 0 : TX + 1;
+    // halo: the 2 TX + 2 TY cells around the tile (rows y0-1 and y0+TY, columns
+    // x0-1 and x0+TX) go to threads 0 .. 2 TX + 2 TY - 1, one cell each; hOK
+    // marks a cell inside the field (the others stay zero in LDS)
+    static_assert(2 * TX + 2 * TY <= NT, "one halo cell per thread");
+    const int hi = threadIdx.x;
+    int hr = 0, hcol = 0, hx = -1, hy = -1;
+    if (hi < TX) {
+        hr = 0, hcol = hi + 1, hx = x0 + hi, hy = y0 - 1;
+    } else if (hi < 2 * TX) {
+        hr = TY + 1, hcol = hi - TX + 1, hx = x0 + hi - TX, hy = y0 + TY;
+    } else if (hi < 2 * TX + TY) {
+        hr = hi - 2 * TX + 1, hcol = 0, hx = x0 - 1, hy = y0 + hi - 2 * TX;
+    } else if (hi < 2 * TX + 2 * TY) {
+        hr = hi - 2 * TX - TY + 1, hcol = TX + 1, hx = x0 + TX, hy = y0 + hi - 2 * TX - TY;
+    }
+    const bool hOK = hx >= 0 && hx < nx && hy >= 0 && hy < ny;
+    const long cH = hOK ? (long)hy * nx + hx : 0;
     for (int i = threadIdx.x; i < 2 * (TY + 2) * (TX + 2); i += NT) (&t[0][0][0])[i] = 0.0;
 
-    // Values in flight stay in their storage type Tu and are widened where
-    // they are used: a conversion next to the load would make the wave wait
-    // for it there.
-    long o = (long)z0 * pl;
-    double um = (valid && z0 > 0) ? (double)up[o - pl + c0] : 0.0;
-    double uc = valid ? (double)up[o + c0] : 0.0;
-    Tu un = (valid && z0 + 1 < nz) ? up[o + pl + c0] : Tu(0);
-    double fv = valid ? (double)fp[o + c0] : 0.0;
-    Tu hn = hN ? up[o + cN] : Tu(0), hs = hS ? up[o + cS] : Tu(0);
-    Tu hw = hW ? up[o + cW] : Tu(0);
+    // Loads run two planes ahead and carry no branch: a plane index past the
+    // field is clamped to its last plane (an in-bounds read), and the value is
+    // replaced by the Dirichlet zero where it is used.  Values in flight stay
+    // in their storage type Tu and are widened (and selected) at their use:
+    // converting or selecting next to the load would make the wave wait there.
+    // The in-flight values live in three register sets that trade roles from
+    // plane to plane (the loop is unrolled by three), so no register holding a
+    // load in flight is ever copied: a copy would wait for the load.
+    const int zl = nz - 1;
+    auto zcl = [&](int zz) { return (long)(zz < 0 ? 0 : (zz > zl ? zl : zz)) * pl; };
+    auto uval = [&](Tu v, int zz) { return (valid && zz >= 0 && zz < nz) ? (double)v : 0.0; };
+    struct Slot {
+        Tu u;       // u at z+1 (centre column)
+        double f;   // f at z
+        Tu h;       // the thread's halo cell at z
+    };
+    auto load_slot = [&](Slot& a, int zu, int zf) {
+        a.u = up[zcl(zu) + c0];
+        a.f = fp[zcl(zf) + c0];
+        a.h = up[zcl(zf) + cH];
+    };
+    const Tu um0 = up[zcl(z0 - 1) + c0], uc0 = up[zcl(z0) + c0];
+    Slot A, B, C;
+    load_slot(A, z0 + 1, z0);
+    load_slot(B, z0 + 2, z0 + 1);
+    double um = uval(um0, z0 - 1), uc = uval(uc0, z0);
     double rr = 0.0, ff = 0.0;
     __syncthreads();
-    for (int z = z0; z < z1; ++z, o += pl) {
+    // one plane: `cur` holds u at z+1, f and halos at z; `nxt` the same one
+    // plane later (in flight); `fre` is refilled for two planes later
+    auto step = [&](const int z, Slot& cur, Slot& nxt, Slot& fre) {
+        (void)nxt;
+        const long o = (long)z * pl;
         auto& b = t[z & 1];
         b[ty + 1][tx + 1] = uc;
-        if (hN) b[0][tx + 1] = (double)hn;
-        if (hS) b[TY + 1][tx + 1] = (double)hs;
-        if (hW) b[ty + 1][wcol] = (double)hw;
-        // in flight while plane z is computed: u at z+2 (plane z+1's upper
-        // neighbour), f and the halos at z+1
-        const bool more = z + 1 < z1;
-        const Tu unn = (valid && more && z + 2 < nz) ? up[o + 2 * pl + c0] : Tu(0);
-        const double fn = (valid && more) ? (double)fp[o + pl + c0] : 0.0;
-        const Tu hn2 = (hN && more) ? up[o + pl + cN] : Tu(0);
-        const Tu hs2 = (hS && more) ? up[o + pl + cS] : Tu(0);
-        const Tu hw2 = (hW && more) ? up[o + pl + cW] : Tu(0);
+        if (hOK) b[hr][hcol] = (double)cur.h;
+        load_slot(fre, z + 3, z + 2);   // in flight while planes z and z+1 are computed
         __syncthreads();   // b complete; the other buffer's readers (plane z-1) are past this point
-        const double udn = (double)un;
+        const double udn = uval(cur.u, z + 1);
+        const double fv = cur.f;
         const double au = (b[ty + 1][tx] - 2.0 * uc + b[ty + 1][tx + 2]) * ix +
                           (b[ty][tx + 1] - 2.0 * uc + b[ty + 2][tx + 1]) * iy + (um - 2.0 * uc + udn) * iz;
         const double r = fv - au;
@@ -215,12 +236,15 @@ __global__ __launch_bounds__(TX* TY) void residual_tile_kernel(
         }
         um = uc;
         uc = udn;
-        un = unn;
-        fv = fn;
-        hn = hn2;
-        hs = hs2;
-        hw = hw2;
+    };
+    int z = z0;
+    for (; z + 3 <= z1; z += 3) {
+        step(z, A, B, C);
+        step(z + 1, B, C, A);
+        step(z + 2, C, A, B);
     }
+    if (z < z1) step(z, A, B, C);
+    if (z + 1 < z1) step(z + 1, B, C, A);
 }
 
 // z-chunk length in granules: whole columns unless the batch is too small to
