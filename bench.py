@@ -443,15 +443,18 @@ def leg_c3(steps: int = 5, count: int = 512, tune_reps: int = 3):
 
 
 def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 32, seed: int = 4,
-           fp64_sample: int = 120):
-    """Serving: Poisson arrivals of the uneven mix + 5% 255^3 (FP32+refine)."""
+           fp64_sample: int = 120, fracs=(0.25, 0.5, 0.75), rates=None, sat_requests: int = 1200):
+    """Serving: Poisson arrivals of the uneven mix + 5% 255^3 (FP32+refine).
+    Saturation is the throughput of a burst of `sat_requests` requests (one
+    255^3 request in twenty carries about three quarters of the mix's points,
+    so the burst must be long for its share to be typical); the Poisson runs
+    offer `fracs` of it, or the absolute `rates` (requests/s) when given."""
     import fdm_amd
     from fdm_amd import hip, serve
     rng = np.random.default_rng(seed)
     depth = 24                   # in-flight requests per shape (own rhs and ans buffers each)
     # request streams first, so every buffer exists before anything is timed
-    warm = uneven_mix(600, seed, big_frac=0.05)
-    fracs = (0.5, 0.8, 1.0)
+    warm = uneven_mix(200 + sat_requests, seed, big_frac=0.05)
     fp64_reqs = uneven_mix(fp64_sample, seed + 7, big_frac=0.05)
     shapes = {}
 
@@ -512,18 +515,19 @@ def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 32
     # warm-up (plans, tables, pinned batches, iteration counts), then saturation:
     # a burst of requests all submitted at t = 0
     run(warm[:200], None)
-    dt, lat, _, _ = run(warm[200:600], None)
-    sat = 400 / dt
-    burst_submit_per_s = 400 / run.submit_s
+    dt, lat, _, _ = run(warm[200:], None)
+    sat = sat_requests / dt
+    burst_submit_per_s = sat_requests / run.submit_s
     out = {"workload": "requests of the uneven mix (each axis n in {31,63,127}, 25% 2-D) + 5% 255^3, "
                        "FP32+refine to 1e-10, Poisson arrivals (seeded exponential gaps)",
            "hbm_budget_gb": budget_gb, "max_batch": max_batch, "streams": 4,
            "saturation_solves_per_s": round(sat, 1), "burst_client_submit_per_s": round(burst_submit_per_s, 1),
            "rates": []}
-    for frac in fracs:
-        rate = frac * sat
+    offered = [(r, 1000 + k) for k, r in enumerate(rates)] if rates is not None else \
+        [(frac * sat, int(frac * 100)) for frac in fracs]
+    for rate, tag in offered:
         n = max(50, int(rate * duration_s))
-        reqs = uneven_mix(n, seed + int(frac * 100), big_frac=0.05)
+        reqs = uneven_mix(n, seed + tag, big_frac=0.05)
         gaps = rng.exponential(1.0 / rate, n)
         dt, lat, resid, ok = run(reqs, gaps)
         out["rates"].append({"offered_per_s": round(rate, 1), "requests": n,
