@@ -143,6 +143,54 @@ def test_cube_refine(fdm, n):
     assert rel2(ans, ref) <= 1e-9
 
 
+# Shapes for the FP64 residual's z march (loads two planes ahead, the loop
+# unrolled by three): z lengths 1, 3, 7, 15, 31 (tails of 1 and 2 planes),
+# x tiles cut by the field edge (255 = 128 + 127, 100-wide rows do not occur:
+# Dirichlet lengths are 2^k - 1), single problems whose columns are split into
+# z chunks, and the axis-tile path (non-square planes).
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 3, 3), (7, 7, 1), (15, 7, 3), (31, 127, 7), (255, 15, 31),
+                                   (63, 63, 15)])
+def test_cube_refine_shapes(fdm, shape):
+    nx, ny, nz = shape
+    say("cube refine shape", shape)
+    g, rhs = cube_setup(nx, ny, nz, seed=9)
+    ref = O.lapl_cube(rhs, *g)
+    s = fdm.LaplCube(*g, nx, ny, nz, precision="refine")
+    ans = np.zeros_like(rhs)
+    s.solve(ans, rhs)
+    st = s.stats()
+    res = O.rel_residual(rhs, ans, g[0], g[1], g[2])
+    print(f"   iterations={st.iterations} device_residual={st.rel_residual:.3e} host_residual={res:.3e} "
+          f"err_vs_fp64={rel2(ans, ref):.3e}", flush=True)
+    assert res <= 1e-10
+    assert st.rel_residual <= 1e-10
+    assert abs(st.rel_residual - res) <= 1e-12 + 0.05 * res
+    assert rel2(ans, ref) <= 1e-9
+
+
+def test_cube_tile_batch_matches_single(fdm):
+    from fdm_amd import hip
+    say("cube batch of 5 on device, axis-tile path, f64 and refine, 31x15x7")
+    nx, ny, nz = 31, 15, 7
+    g, _ = cube_setup(nx, ny, nz)
+    fs = np.stack([cube_setup(nx, ny, nz, seed=20 + k)[1] for k in range(5)])
+    for prec in ("f64", "refine"):
+        s = fdm.LaplCube(*g, nx, ny, nz, precision=prec)
+        rhs = hip.DeviceArray.from_host(fs)
+        out = hip.DeviceArray(fs.shape, np.float64)
+        s.solve_batch(out, rhs)
+        hip.synchronize()
+        batch = out.download()
+        for i in (0, 3, 4):
+            one_r = hip.DeviceArray.from_host(fs[i])
+            one = hip.DeviceArray(fs[i].shape, np.float64)
+            s.solve(one, one_r)
+            hip.synchronize()
+            assert np.array_equal(one.download(), batch[i])
+        ref = O.lapl_cube(fs[4], *g)
+        assert rel2(batch[4], ref) <= (1e-12 if prec == "f64" else 1e-9)
+
+
 def test_cube_device_batch_matches_single(fdm):
     from fdm_amd import hip
     say("cube batch of 3 on device, f64 and refine, 63^3")
