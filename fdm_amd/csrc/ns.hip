@@ -173,12 +173,17 @@ __global__ void ns_rhs_kernel(Grid g, Fields f) {
     double r = ((Fa[g.F(i, k, j)] - Fa[g.F(i, k, j - 1)]) / g.dx + (Ga[g.G(i, k, j)] - Ga[g.G(i, k - 1, j)]) / g.dy +
                 (Ha[g.H(i, k, j)] - Ha[g.H(i - 1, k, j)]) / g.dz) /
                g.dt;
-    if (i <= 1) r -= P[g.p(i - 1, k, j)] / dz2;
-    if (k <= 1) r -= P[g.p(i, k - 1, j)] / dy2;
-    if (j <= 1) r -= P[g.p(i, k, j - 1)] / dx2;
-    if (j >= g.nx) r -= P[g.p(i, k, j + 1)] / dx2;
-    if (k >= g.ny) r -= P[g.p(i, k + 1, j)] / dy2;
-    if (i >= g.nz) r -= P[g.p(i + 1, k, j)] / dz2;
+    // p has a ghost layer on every side, so the six neighbours of an interior
+    // point are always in the array: load them all (no branch around a load)
+    // and keep the boundary ones
+    const double pzm = P[g.p(i - 1, k, j)], pym = P[g.p(i, k - 1, j)], pxm = P[g.p(i, k, j - 1)];
+    const double pxp = P[g.p(i, k, j + 1)], pyp = P[g.p(i, k + 1, j)], pzp = P[g.p(i + 1, k, j)];
+    if (i <= 1) r -= pzm / dz2;
+    if (k <= 1) r -= pym / dy2;
+    if (j <= 1) r -= pxm / dx2;
+    if (j >= g.nx) r -= pxp / dx2;
+    if (k >= g.ny) r -= pyp / dy2;
+    if (i >= g.nz) r -= pzp / dz2;
     as_global(f.RHS)[g.x(i, k, j)] = r;
 }
 
@@ -190,10 +195,17 @@ __global__ void ns_update_kernel(Grid g, Fields f) {
     const int i = blockIdx.z + 1;
     if (j > g.nx || k > g.ny) return;
     const auto X = as_global((const double*)f.x);
+    // every load is issued before the first use (no branch around a load):
+    // the x neighbours are clamped to the last interior point (their value
+    // is used only below the bound); F, G and H hold the face at index n in
+    // their own direction, so F(i,k,nx), G(i,ny,j), H(nz,k,j) are in the array
     const double xc = X[g.x(i, k, j)];
-    if (j < g.nx) as_global(f.u)[g.u(i, k, j)] = f.F[g.F(i, k, j)] - g.dt / g.dx * (X[g.x(i, k, j + 1)] - xc);
-    if (k < g.ny) as_global(f.v)[g.v(i, k, j)] = f.G[g.G(i, k, j)] - g.dt / g.dy * (X[g.x(i, k + 1, j)] - xc);
-    if (i < g.nz) as_global(f.w)[g.w(i, k, j)] = f.H[g.H(i, k, j)] - g.dt / g.dz * (X[g.x(i + 1, k, j)] - xc);
+    const double xj = X[g.x(i, k, j < g.nx ? j + 1 : j)], xk = X[g.x(i, k < g.ny ? k + 1 : k, j)];
+    const double xi = X[g.x(i < g.nz ? i + 1 : i, k, j)];
+    const double fu = f.F[g.F(i, k, j)], fv = f.G[g.G(i, k, j)], fw = f.H[g.H(i, k, j)];
+    if (j < g.nx) as_global(f.u)[g.u(i, k, j)] = fu - g.dt / g.dx * (xj - xc);
+    if (k < g.ny) as_global(f.v)[g.v(i, k, j)] = fv - g.dt / g.dy * (xk - xc);
+    if (i < g.nz) as_global(f.w)[g.w(i, k, j)] = fw - g.dt / g.dz * (xi - xc);
     as_global(f.p)[g.p(i, k, j)] = xc;
 }
 
