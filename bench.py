@@ -341,6 +341,31 @@ def run_gpu(mode: str, n: int, batch: int, steps: int, warmup: int, f_host: np.n
     }
 
 
+def leg_host(n: int = 127, reps: int = 10):
+    """The drop-in's host-array solve (fdmgpu_solve_host: pageable host arrays
+    in and out, as LaplCube::solve(ans, rhs) receives them), PCIe copies
+    included; one problem per call."""
+    import fdm_amd
+    u, f = manufactured_shape("cube", n, n, n, 5)
+    g = shape_geometry("cube", n, n, n)
+    out = {"workload": f"LaplCube Dirichlet {n}^3, host arrays (pageable) in and out, one solve per call"}
+    for mode in ("f64", "refine"):
+        s = fdm_amd.LaplCube(*g, n, n, n, precision=mode)
+        if mode == "refine":
+            s.set_refine(1e-10, 2)
+        ans = np.zeros_like(f)
+        s.solve(ans, f)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.solve(ans, f)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[mode] = {"ms_per_solve": round(ms, 3), "solves_per_s": round(1e3 / ms, 1),
+                     "err_vs_manufactured": float(np.linalg.norm(ans - u) / np.linalg.norm(u)),
+                     "pcie_bytes_per_solve": 2 * f.nbytes}
+        del s
+    return out
+
+
 def leg_c2(reps: int = 5):
     """One 255^3 problem: plain FP64 vs FP32+refine (axis-tile kernels: a 256^2
     plane does not fit LDS)."""
@@ -701,8 +726,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mode", choices=("both", "refine", "f64"), default="both",
                     help="which GPU legs to run (profiling runs use one)")
-    ap.add_argument("--legs", default="head,c2,c3,c4,c5,pub,cpu",
-                    help="comma list of legs: head (the value; always run), c2, c3, c4, c5, pub, cpu")
+    ap.add_argument("--legs", default="head,c2,c3,c4,c5,pub,host,cpu",
+                    help="comma list of legs: head (the value; always run), c2, c3, c4, c5, pub, host, cpu")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving rate")
     ap.add_argument("--profile-summary", nargs=2, metavar=("KT_DIR", "PMC_DIR"),
                     help="summarise rocprofv3 csv output into profiles/ and exit (no GPU)")
@@ -813,6 +838,9 @@ def main():
         out["serve"] = leg_c4(duration_s=args.serve_seconds)
         out["p50_ms"] = [r["p50_ms"] for r in out["serve"]["rates"]]
         out["p99_ms"] = [r["p99_ms"] for r in out["serve"]["rates"]]
+    if "host" in legs:
+        progress("host: drop-in host-array solves, PCIe copies included")
+        out["host_solve"] = leg_host()
     if "pub" in legs:
         progress("pub: periodic LaplCube 32^3..128^3 one at a time (the reference's published Poisson stage)")
         out["published_poisson"] = leg_published()
