@@ -101,8 +101,9 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
 // y in [y0, y0+TY) of one z-chunk and marches through it.  The u plane of the
 // current z sits in LDS (two buffers, one barrier per plane) with a one-point
 // halo, so the x / y neighbours are LDS reads; u at z-1, z, z+1 stay in
-// registers.  Loads run ahead of use: while plane z is computed, u at z+2
-// and f and the halo values at z+1 are in flight.  Each u value is read from
+// registers.  Loads run two planes ahead of use: while plane z is computed,
+// u at z+2 and z+3 and f and the halo values at z+1 and z+2 are in flight
+// (memory-level parallelism is what bounds this stream).  Each u value is read from
 // HBM once per chunk plus the chunk's two boundary planes and the tile's halo
 // rows.  Tiles are as wide as the rows where possible (no x halo).
 // Blocks go round-robin over the 8 XCDs; the logical block index (problem,
