@@ -127,12 +127,17 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, N / 2, LY, NT>(lds, L, sy, tid);   // pencils = columns (y)
     __syncthreads();
-    plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) {
-        const T v = lds[LX::at(IO::out(y), IO::out(x))];
-        if constexpr (ACC == 1) dst[y * n + x] += (Tout)v;
-        else if constexpr (ACC == 2) dst[y * n + x] = (Tout)bp[y * n + x] + (Tout)v;
-        else dst[y * n + x] = (Tout)v;
-    });
+    if constexpr (ACC == 2) {   // the FP32 base: every load in flight before the first store
+        plane_stage<LOGN, NT, n, float>(
+            tid, [&](int y, int x) { return bp[y * n + x]; },
+            [&](int y, int x, float b) { dst[y * n + x] = (Tout)b + (Tout)lds[LX::at(IO::out(y), IO::out(x))]; });
+    } else {
+        plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) {
+            const T v = lds[LX::at(IO::out(y), IO::out(x))];
+            if constexpr (ACC == 1) dst[y * n + x] += (Tout)v;
+            else dst[y * n + x] = (Tout)v;
+        });
+    }
 }
 
 // ---------------------------------------------------------------------------
