@@ -3,7 +3,18 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace fdmgpu {
+
+// Compile-time loop: f(integral_constant<int, i>) for i = B .. E-1.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
 // Where problem p of a batch lives: base + p*bs (problems back to back), or
 // tab[p] when a device table of per-problem pointers is given (problems in

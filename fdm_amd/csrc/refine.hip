@@ -102,8 +102,7 @@ __global__ __launch_bounds__(kResBlock) void residual_kernel(
 // current z sits in LDS (two buffers, one barrier per plane) with a one-point
 // halo, so the x / y neighbours are LDS reads; u at z-1, z, z+1 stay in
 // registers.  Loads run two planes ahead of use: while plane z is computed,
-// u at z+2 and z+3 and f and the halo values at z+1 and z+2 are in flight
-// (memory-level parallelism is what bounds this stream).  Each u value is read from
+// u at z+2 and z+3 and f and the halo values at z+1 and z+2 are in flight.  Each u value is read from
 // HBM once per chunk plus the chunk's two boundary planes and the tile's halo
 // rows.  Tiles are as wide as the rows where possible (no x halo).
 // Blocks go round-robin over the 8 XCDs; the logical block index (problem,
@@ -173,13 +172,13 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(8))) voi
     const long cH = hOK ? (long)hy * nx + hx : 0;
     for (int i = threadIdx.x; i < 2 * (TY + 2) * (TX + 2); i += NT) (&t[0][0][0])[i] = 0.0;
 
-    // Loads run two planes ahead and carry no branch: a plane index past the
+    // Loads run D planes ahead and carry no branch: a plane index past the
     // field is clamped to its last plane (an in-bounds read), and the value is
     // replaced by the Dirichlet zero where it is used.  Values in flight stay
     // in their storage type Tu and are widened (and selected) at their use:
     // converting or selecting next to the load would make the wave wait there.
-    // The in-flight values live in three register sets that trade roles from
-    // plane to plane (the loop is unrolled by three), so no register holding a
+    // The in-flight values live in D + 1 register sets that trade roles from
+    // plane to plane (the loop is unrolled by D + 1), so no register holding a
     // load in flight is ever copied: a copy would wait for the load.
     const int zl = nz - 1;
     auto zcl = [&](int zz) { return (long)(zz < 0 ? 0 : (zz > zl ? zl : zz)) * pl; };
@@ -194,22 +193,26 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(8))) voi
         a.f = fp[zcl(zf) + c0];
         a.h = up[zcl(zf) + cH];
     };
+    // prefetch depth in planes (three measured no faster than two for the
+    // read-only FP64 pass: its memory parallelism is not what bounds it)
+    constexpr int D = 2, NS = D + 1;
     const Tu um0 = up[zcl(z0 - 1) + c0], uc0 = up[zcl(z0) + c0];
-    Slot A, B, C;
-    load_slot(A, z0 + 1, z0);
-    load_slot(B, z0 + 2, z0 + 1);
+    Slot sl[NS];
+    static_for<0, D>([&](auto kk) {
+        constexpr int q = decltype(kk)::value;
+        load_slot(sl[q], z0 + q + 1, z0 + q);
+    });
     double um = uval(um0, z0 - 1), uc = uval(uc0, z0);
     double rr = 0.0, ff = 0.0;
     __syncthreads();
-    // one plane: `cur` holds u at z+1, f and halos at z; `nxt` the same one
-    // plane later (in flight); `fre` is refilled for two planes later
-    auto step = [&](const int z, Slot& cur, Slot& nxt, Slot& fre) {
-        (void)nxt;
+    // one plane: `cur` holds u at z+1, f and halos at z; the next D-1 sets are
+    // in flight; `fre` is refilled for D planes later
+    auto step = [&](const int z, Slot& cur, Slot& fre) {
         const long o = (long)z * pl;
         auto& b = t[z & 1];
         b[ty + 1][tx + 1] = uc;
         if (hOK) b[hr][hcol] = (double)cur.h;
-        load_slot(fre, z + 3, z + 2);   // in flight while planes z and z+1 are computed
+        load_slot(fre, z + D + 1, z + D);   // in flight while planes z .. z+D-1 are computed
         __syncthreads();   // b complete; the other buffer's readers (plane z-1) are past this point
         const double udn = uval(cur.u, z + 1);
         const double fv = cur.f;
@@ -239,13 +242,15 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(8))) voi
         uc = udn;
     };
     int z = z0;
-    for (; z + 3 <= z1; z += 3) {
-        step(z, A, B, C);
-        step(z + 1, B, C, A);
-        step(z + 2, C, A, B);
-    }
-    if (z < z1) step(z, A, B, C);
-    if (z + 1 < z1) step(z + 1, B, C, A);
+    for (; z + NS <= z1; z += NS)
+        static_for<0, NS>([&](auto kk) {
+            constexpr int q = decltype(kk)::value;
+            step(z + q, sl[q], sl[(q + D) % NS]);
+        });
+    static_for<0, NS - 1>([&](auto kk) {   // the last z1 - z < NS planes
+        constexpr int q = decltype(kk)::value;
+        if (z + q < z1) step(z + q, sl[q], sl[(q + D) % NS]);
+    });
 }
 
 // z-chunk length in granules: whole columns unless the batch is too small to
