@@ -167,6 +167,15 @@ int default_tile(int logn) {
     return tile_ok<T>(logn, 32) ? 32 : 16;
 }
 
+// y tiles of FP32 planes at least 128 wide: 64 pencils when two such blocks
+// fit a CU (measured 4% faster than 32 for the 255^3 FP32+refine solve)
+template <typename T>
+int default_tile_y(int logn, int nx) {
+    if (sizeof(T) == 4 && nx >= 128 && tile_ok<T>(logn, 64) && 2 * tile_lds_bytes(logn, 64, 4) <= kLdsBytes)
+        return 64;
+    return default_tile<T>(logn);
+}
+
 template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND, int TILE>
 static hipError_t xaxis_k(const AxisArgs& a) {
     using C = TileCfg<LOGN, T, TILE>;
@@ -236,7 +245,7 @@ static hipError_t yaxis_tile(const AxisArgs& a) {
 
 template <typename Tin, typename Tout, typename T, int LOGN>
 static hipError_t yaxis_t(const AxisArgs& a) {
-    const int tile = a.tile ? a.tile : default_tile<T>(LOGN);
+    const int tile = a.tile ? a.tile : default_tile_y<T>(LOGN, a.nx);
     if (!tile_ok<T>(LOGN, tile)) return hipErrorInvalidValue;
     if (tile == 16) return yaxis_tile<Tin, Tout, T, LOGN, 16>(a);
     if constexpr (LOGN >= 5) {

@@ -586,13 +586,15 @@ static hipError_t zcr_strip_k(const ZcrArgs& a) {
 template <typename Ts, typename T, int Q>
 static hipError_t zcr_g(const ZcrArgs& a) {
     // default: Dirichlet systems in the strip layout (64 consecutive modes per
-    // wave) with 16-row chunks (16 to 256 rows; measured fastest at 128 rows:
-    // 8 waves per SIMD in FP32), 512 rows with 32-row chunks; circulant ones
-    // with 2 lanes per 64 rows from 64 rows up (fewer registers per lane,
-    // twice the occupancy of 1)
-    const int gm = a.gmul                                   ? a.gmul
-                   : (!a.periodic && zcr_gmul_ok(Q, 5)) ? 5
-                   : (!a.periodic && zcr_gmul_ok(Q, 3)) ? 3
+    // wave) with 16-row chunks up to 128 rows (measured fastest at 128 rows:
+    // 8 waves per SIMD in FP32) and 32-row chunks from 256 rows (measured 2-3%
+    // faster than 16 at 256 rows, FP64 and FP32); circulant ones with 2 lanes
+    // per 64 rows from 64 rows up (fewer registers per lane, twice the
+    // occupancy of 1)
+    const int gm = a.gmul                                             ? a.gmul
+                   : (!a.periodic && Q <= 7 && zcr_gmul_ok(Q, 5)) ? 5
+                   : (!a.periodic && zcr_gmul_ok(Q, 3))           ? 3
+                   : (!a.periodic && zcr_gmul_ok(Q, 5))           ? 5
                    : (Q >= 6 ? 2 : 1);
     if (!zcr_gmul_ok(Q, gm) || (gm >= 3 && a.periodic)) return hipErrorInvalidValue;
     if (gm == 3) return zcr_strip_k<Ts, T, Q, 1, 32>(a);
