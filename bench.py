@@ -674,6 +674,21 @@ def leg_c5(n: int = 127, steps: int = 1000, check_at: int = 100, cpu_check: bool
                 "last_solve_iterations": s.iterations, "u_max": float(np.abs(u[1:n + 1]).max()),
                 "finite": bool(np.isfinite(u).all())})
     g.close()
+    # beside it: the same run with the plain FP64 pressure solve (what the
+    # drop-in LaplCube<double> uses by default), and its distance to the
+    # refined run after the same number of steps
+    g64 = fdm_amd.NSCube(n, n, n, solver="f64", **kw)
+    g64.step(check_at, st)
+    st.synchronize()
+    e0.record(st)
+    g64.step(rest, st)
+    e1.record(st)
+    e1.synchronize()
+    ms64 = e0.elapsed_ms(e1)
+    u64 = g64.get("u")
+    out["fp64_solver"] = {"steps_per_s": round(rest / (ms64 / 1e3), 1), "ms_per_step": round(ms64 / rest, 4),
+                          "u_max_rel_diff_vs_refine": float(np.abs(u64 - u).max() / np.abs(u).max())}
+    g64.close()
     return out
 
 
