@@ -71,9 +71,12 @@ def manufactured(shape, seed):
     return u, f
 
 
+# (255, 127) refine and (255, 31, 15) f32: FP32 y tiles of rows at least 128
+# wide, 64 pencils per block by default
 @pytest.mark.parametrize("case", [("cube", (31, 31, 31), "refine"), ("cube", (63, 31, 15), "f64"),
                                   ("cube", (63, 63, 63), "f32"), ("cube", (127, 127, 31), "refine"),
-                                  ("rect", (100, 63), "refine"), ("rect", (255, 127), "f64")])
+                                  ("rect", (100, 63), "refine"), ("rect", (255, 127), "f64"),
+                                  ("rect", (255, 127), "refine"), ("cube", (255, 31, 15), "f32")])
 def test_every_variant_parity(fdm, case):
     kind, shape, prec = case
     say("variants", case)
