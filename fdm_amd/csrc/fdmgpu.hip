@@ -223,7 +223,9 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
     const bool refine = P->precision == FDMGPU_F32_REFINE64;
     for (int gm : gms) {
         if (plane_capable(P)) {
-            for (int fu : (refine ? std::vector<int>{1, 0} : std::vector<int>{1})) {
+            // fuse selects the refinement's fused residual pass: plain plans
+            // list 0 only, the value of their default
+            for (int fu : (refine ? std::vector<int>{1, 0} : std::vector<int>{0})) {
                 Variant v;
                 v.plane = 1;
                 v.zcr_gmul = gm;
