@@ -132,10 +132,13 @@ int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_ta
  * wavefront per 32-row chunk of 64 consecutive modes, 64 (3) or 128 (4)
  * modes per block (Dirichlet z, 32 to 512 rows; 4 up to 128), 5 = the same
  * with 16-row chunks (16 to 256 rows); 0 = default
- * (5, else 3 where valid, else 2 from 64 rows, else 1); fuse (FP32+refine plane
- * path) = FP64 residual fused into the next forward transform (1) or a
- * separate residual pass (0, the default).  Every variant solves the same system; results
- * differ at rounding level.  No reference counterpart. */
+ * (Dirichlet: 5 up to 128 rows, 3 from 256 rows, else 5 where valid;
+ * circulant: 2 from 64 rows, else 1); tile_y = 0 takes 64 for FP32 planes at
+ * least 128 wide when two such blocks fit a CU, else 32 (16 where 32 does not
+ * fit); fuse (FP32+refine plane path) = FP64 residual fused into the next
+ * forward transform (1) or a separate residual pass (0, the default; plain
+ * plans list 0 only).  Every variant solves the same system; results differ
+ * at rounding level.  No reference counterpart. */
 typedef struct fdmgpu_variant {
     int plane, tile_x, tile_y, zcr_gmul;
     int fuse; /* FP32+refine on the plane path: 1 = residual fused into the forward transform */
