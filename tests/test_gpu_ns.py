@@ -3,7 +3,8 @@
 Bars: with the FP64 pressure solve the velocity and pressure fields match the
 oracle to 1e-11 relative (max-norm over each field) after tens of steps; with
 the FP32+refine pressure solve (residual 1e-10 per step) to 1e-8 after 100
-steps, the north star's bar for the cavity.  Steps after the second are
+steps (the cavity's bar: FP64 CPU agreement to 1e-8 relative at step 100).
+Steps after the second are
 graph replays, and the comparisons are made between step batches (host
 copies in between).
 """

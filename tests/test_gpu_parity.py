@@ -2,8 +2,8 @@
 reference's golden vectors.
 
 Tolerances (stated per test):
-  * FP64 plans: ||u_gpu - u_cpu||_2 / ||u_cpu||_2 <= 1e-12 (the north-star
-    contract "plain GPU FP64 must match the CPU path to <= 1e-12 relative");
+  * FP64 plans: ||u_gpu - u_cpu||_2 / ||u_cpu||_2 <= 1e-12 (plain GPU FP64
+    must match the CPU path to <= 1e-12 relative);
   * FP32 plans: relative L2 error vs the FP64 and FP32 oracles <= 2e-6 (float
     rounding; the reference's own float path is ~4e-7 from the FP64 answer),
     and <= 2e-5 vs the reference's float golden vectors;
