@@ -253,7 +253,15 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s) {
     NS_HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(ns_rhs_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
     NS_HIPCHK(hipGetLastError());
-    int rc = plan_solve(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s);
+    // FP32+refine: from the second step on, x still holds the previous step's
+    // pressure, a close iterate for the new right-hand side; the refinement
+    // continues from it (FP64 residual, then FP32 corrections: inner solves
+    // 2 and, as a guard that costs only empty launches once converged, 3)
+    // instead of solving from zero.  The tolerance and the stopping test are
+    // those of a cold solve.
+    const bool warm = N->plan->precision == FDMGPU_F32_REFINE64 && N->steps > 0;
+    int rc = warm ? plan_solve_iters(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s, 2, 3)
+                  : plan_solve(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s);
     if (rc) return rc;
     hipLaunchKernelGGL(ns_update_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
     NS_HIPCHK(hipGetLastError());
