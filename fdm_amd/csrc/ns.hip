@@ -222,6 +222,7 @@ struct fdmgpu_ns_t {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     long steps = 0;
+    bool warm_ok = false;          // x holds the pressure of the previous step (a field upload clears it)
     bool use_graph = true;
 };
 
@@ -259,7 +260,7 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s) {
     // 2 and, as a guard that costs only empty launches once converged, 3)
     // instead of solving from zero.  The tolerance and the stopping test are
     // those of a cold solve.
-    const bool warm = N->plan->precision == FDMGPU_F32_REFINE64 && N->steps > 0;
+    const bool warm = N->plan->precision == FDMGPU_F32_REFINE64 && N->warm_ok;
     int rc = warm ? plan_solve_iters(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s, 2, 3)
                   : plan_solve(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s);
     if (rc) return rc;
@@ -344,7 +345,9 @@ int fdmgpu_ns_step(fdmgpu_ns N, int count, void* stream) {
     NS_HIPCHK(hipEventRecord(N->ev_in, user));
     NS_HIPCHK(hipStreamWaitEvent(s, N->ev_in, 0));
     for (int c = 0; c < count; c++) {
-        if (!N->use_graph || N->steps < 2) {   // eager for the first two steps (allocations, warm-up)
+        // eager for the first two steps (allocations, warm-up) and for a step
+        // whose solve starts from zero after a field upload; replayed after
+        if (!N->use_graph || N->steps < 2 || !N->warm_ok) {
             int rc = ns_enqueue_step(N, s);
             if (rc) return rc;
         } else {
@@ -364,6 +367,7 @@ int fdmgpu_ns_step(fdmgpu_ns N, int count, void* stream) {
             NS_HIPCHK(hipGraphLaunch(N->exec, s));
         }
         N->steps++;
+        N->warm_ok = true;
     }
     if (N->steps > 0) {   // fdmgpu_get_stats on the solver plan reports the last step's pressure solve
         N->plan->last_stream = s;
@@ -391,6 +395,7 @@ int fdmgpu_ns_set(fdmgpu_ns N, int which, const double* host, unsigned long long
     NS_HIPCHK(hipSetDevice(N->device));
     NS_HIPCHK(hipStreamSynchronize(N->cap));
     NS_HIPCHK(hipMemcpy(*p, host, sizeof(double) * count, hipMemcpyHostToDevice));
+    N->warm_ok = false;   // the next pressure solve starts from zero
     return FDMGPU_OK;
 }
 
