@@ -258,12 +258,12 @@ def profile_summary(kt: str, pmc: str, n: int = 127, batch: int = 32):
         "FP32+refine to 1e-10 (2 inner solves, default kernel variants) and plain FP64.\n",
         "Commands (from the repository root, one MI355X; the last one writes this file and the json beside it):\n",
         "```",
-        "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/kt -o kt -- python3 bench.py --steps 5 --warmup 1 --no-cpu --legs head",
-        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmcA/FETCH_SIZE -o FETCH_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
-        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmcA/WRITE_SIZE -o WRITE_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
-        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmcA/copy_FETCH_SIZE -o copy_FETCH_SIZE -- python3 bench.py --calib-copy-mib 256",
-        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmcA/copy_WRITE_SIZE -o copy_WRITE_SIZE -- python3 bench.py --calib-copy-mib 256",
-        "python3 bench.py --profile-summary gpurun_out/kt gpurun_out/pmcA",
+        f"rocprofv3 --kernel-trace --stats --output-format csv -d {kt} -o kt -- python3 bench.py --steps 5 --warmup 1 --no-cpu --legs head",
+        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {pmc}/FETCH_SIZE -o FETCH_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
+        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {pmc}/WRITE_SIZE -o WRITE_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
+        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {pmc}/copy_FETCH_SIZE -o copy_FETCH_SIZE -- python3 bench.py --calib-copy-mib 256",
+        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {pmc}/copy_WRITE_SIZE -o copy_WRITE_SIZE -- python3 bench.py --calib-copy-mib 256",
+        f"python3 bench.py --profile-summary {kt} {pmc}",
         "```\n",
         f"Calibration: a 256 MiB device-to-device copy reads FETCH_SIZE = {copy_kib / kf:.0f} KiB and WRITE_SIZE = "
         f"{copy_kib / kw:.0f} KiB, so FETCH_SIZE is scaled by {kf:.2f} and WRITE_SIZE by {kw:.2f} to get bytes.\n",
