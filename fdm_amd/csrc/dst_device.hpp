@@ -9,6 +9,7 @@
 // k = 1..N-1, N = 2^LOGN.  Algorithm (not the reference's Samarskii-Nikolaev
 // recursion, which is sequential per pencil):
 //   1. w_j = sin(pi j/N)(x_j + x_{N-j}) + (x_j - x_{N-j})/2        (w_0 = 0)
+//          = (sin(pi j/N) + 1/2)(x_j + x_{N-j}) - x_{N-j}   (one add, one fma)
 //      Two pencils a, b are packed as z_j = w^a_j + i w^b_j.
 //   2. Z = FFT_N(z), done as a 4-step FFT N = N1 x N2: N1 lanes per job each
 //      run an N2-point DFT in registers, twiddle by w_N^{j1 k2}, exchange
@@ -148,19 +149,20 @@ struct DstShape {
 };
 
 // Per-lane constants of the DST (the lane's index t inside its job is fixed for
-// the whole kernel): twiddles w_N^{t k2}, k2 < N2, and sines sin(pi (t+N1 j2)/N).
+// the whole kernel): twiddles w_N^{t k2}, k2 < N2, and shifted sines
+// sin(pi (t+N1 j2)/N) + 1/2.
 template <typename T, int LOGN>
 struct DstLane {
     using S = DstShape<LOGN>;
-    T twr[S::N2], twi[S::N2], sn[S::N2];
-    // tables (global memory): cos(2 pi m/N), -sin(2 pi m/N), sin(pi m/N), m < N
+    T twr[S::N2], twi[S::N2], sp[S::N2];
+    // tables (global memory): cos(2 pi m/N), -sin(2 pi m/N), sin(pi m/N) + 1/2, m < N
     __device__ __forceinline__ void init(const T* __restrict__ tables, const int t) {
         constexpr int N = S::N, N1 = S::N1, N2 = S::N2;
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
             twr[k2] = tables[t * k2];
             twi[k2] = tables[N + t * k2];
-            sn[k2] = tables[2 * N + t + N1 * k2];
+            sp[k2] = tables[2 * N + t + N1 * k2];
         }
     }
 };
@@ -306,9 +308,9 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                     const T a0 = PA(t, N1 * j2), a1 = PA(jl, jc);
                     const T b0 = PB(t, N1 * j2), b1 = PB(jl, jc);
                     if constexpr (KIND == kDst) {
-                        const T s = L.sn[j2];
-                        yr[j2] = s * (a0 + a1) + T(0.5) * (a0 - a1);
-                        yi[j2] = s * (b0 + b1) + T(0.5) * (b0 - b1);
+                        const T s = L.sp[j2];
+                        yr[j2] = fma(s, a0 + a1, -a1);
+                        yi[j2] = fma(s, b0 + b1, -b1);
                     } else if (2 * j2 < N2) {   // 0 <= j < N/2
                         const bool edge = j == 0;
                         yr[j2] = edge ? T(0.5) * a0 : T(0.5) * (a0 + b1);
@@ -394,7 +396,10 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                 }
                 wave_lds_sync();
             } else if constexpr (KIND == kDst) {
-                // ---- step 3: even outputs directly, odd outputs by prefix sum
+                // ---- step 3: even outputs directly, odd outputs by prefix sum.
+                // The 1/2 of both formulas is applied with the scale at the
+                // store (a power of two: the results are bitwise the same).
+                const T hs = T(0.5) * scale;
                 T ae[KC], ao[KC], be[KC], bo[KC];
                 T runA = T(0), runB = T(0);
 #pragma unroll
@@ -403,10 +408,10 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                     const int pk = S::posZ(k), pn = S::posZ((N - k) & (N - 1));
                     const T zkr = PA(pk, 0), zki = PB(pk, 0);
                     const T znr = PA(pn, 0), zni = PB(pn, 0);
-                    ae[i] = T(0.5) * (zni - zki);
-                    be[i] = T(0.5) * (zkr - znr);
-                    T dA = T(0.5) * (zkr + znr);
-                    T dB = T(0.5) * (zki + zni);
+                    ae[i] = zni - zki;
+                    be[i] = zkr - znr;
+                    T dA = zkr + znr;
+                    T dB = zki + zni;
                     if (k == 0) {
                         dA *= T(0.5);
                         dB *= T(0.5);
@@ -431,14 +436,14 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                     // sigma(2k + e) = t*N2 + ((2i + e) ^ t) for k = t*KC + i (2i + e < N2, t < N1):
                     // lane part t*N2 + (((2i + e) mod N1) ^ t), the rest a multiple of N1
                     const int k = t * KC + i;
-                    const T e_a = k == 0 ? T(0) : ae[i] * scale;
-                    const T e_b = k == 0 ? T(0) : be[i] * scale;
+                    const T e_a = k == 0 ? T(0) : ae[i] * hs;
+                    const T e_b = k == 0 ? T(0) : be[i] * hs;
                     const int me = 2 * i, mo = 2 * i + 1;
                     const int le = t * N2 + ((me & (N1 - 1)) ^ t), lo = t * N2 + ((mo & (N1 - 1)) ^ t);
                     PA(le, me & ~(N1 - 1)) = e_a;
                     PB(le, me & ~(N1 - 1)) = e_b;
-                    PA(lo, mo & ~(N1 - 1)) = ao[i] * scale;
-                    PB(lo, mo & ~(N1 - 1)) = bo[i] * scale;
+                    PA(lo, mo & ~(N1 - 1)) = ao[i] * hs;
+                    PB(lo, mo & ~(N1 - 1)) = bo[i] * hs;
                 }
                 wave_lds_sync();
             }

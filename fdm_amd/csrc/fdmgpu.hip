@@ -60,14 +60,15 @@ int upload(void** dst, const std::vector<double>& v) {
     return FDMGPU_OK;
 }
 
-// twr[m] = cos(2 pi m/N), twi[m] = -sin(2 pi m/N), sn[j] = sin(pi j/N); m, j < N
+// twr[m] = cos(2 pi m/N), twi[m] = -sin(2 pi m/N), sp[j] = sin(pi j/N) + 1/2
+// (the DST pre-processing weight, rounded once); m, j < N
 std::vector<double> dst_tables(int N) {
     std::vector<double> t(3 * (size_t)N);
     for (int m = 0; m < N; m++) {
         const double a = 2.0 * M_PI * m / N;
         t[m] = std::cos(a);
         t[N + m] = -std::sin(a);
-        t[2 * N + m] = std::sin(M_PI * m / N);
+        t[2 * N + m] = std::sin(M_PI * m / N) + 0.5;
     }
     return t;
 }
