@@ -40,6 +40,33 @@ __device__ __forceinline__ global_ptr<T> as_global(T* p) {
     return (global_ptr<T>)p;
 }
 
+// A range of global memory addressed through a buffer resource: the byte
+// offset of an access is a per-lane VGPR part plus a wave-uniform SGPR part,
+// so element offsets that are compile-time constants cost no vector
+// instruction (a 64-bit global address costs two per 4 KB window).  Accesses
+// at offsets >= `bytes` read 0 and are dropped on store.  The third word
+// 0x00020000 is the raw 32-bit data format of the gfx9 family (gfx950).
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+struct BufRange {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ BufRange(const T* p, unsigned bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000)) {}
+    __device__ __forceinline__ T load(int voff, int soff) const {
+        if constexpr (sizeof(T) == 8)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    }
+    __device__ __forceinline__ void store(T v, int voff, int soff) const {
+        if constexpr (sizeof(T) == 8)
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    }
+};
+
 // Pencil transform kinds (see pencil_pairs in dst_device.hpp): DST-I on
 // Dirichlet axes, half-complex real DFT forward / inverse on periodic axes.
 enum : int { kDst = 0, kRdftFwd = 1, kRdftInv = 2 };

@@ -52,7 +52,7 @@ __device__ __forceinline__ void plane_sweep(const int tid, F&& f) {
 #pragma unroll UNROLL
         for (int i = 0; i < ITER; ++i) {
             const int y = r0 + R * i;
-            if (R * i + R <= NN || y < NN) f(y, x);
+            if (R * i + R <= NN || y < NN) f(i, y, x);
         }
     }
 }
@@ -72,16 +72,37 @@ __device__ __forceinline__ void plane_stage(const int tid, LD&& ld, ST&& st) {
 #pragma unroll
         for (int i = 0; i < ITER; ++i) {
             const int y = r0 + R * i;
-            if (R * i + R <= NN || y < NN) v[i] = ld(y, x);
+            if (R * i + R <= NN || y < NN) v[i] = ld(i, y, x);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < ITER; ++i) {
             const int y = r0 + R * i;
-            if (R * i + R <= NN || y < NN) st(y, x, v[i]);
+            if (R * i + R <= NN || y < NN) st(i, y, x, v[i]);
         }
     }
 }
+
+constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+
+// LDS slots of a sweep's elements: element i of thread tid is slot
+// (r0 + R*i + o, x + o) of a PlaneLayout (x = tid % N, r0 = tid / N,
+// R = NT / N).  The row swizzle f() has period N1 in the row, so rows a
+// multiple of lcm(R, N1) apart share it: PER bases per thread, and every
+// element is a base plus a compile-time offset (no per-element arithmetic).
+template <int LOGN, int NT, int P, int N1, int O>
+struct SweepSlots {
+    static constexpr int N = 1 << LOGN, R = NT / N;
+    static constexpr int PER = N1 / gcd_int(R, N1);
+    using Lay = PlaneLayout<P, N1, true>;
+    int b[PER];
+    __device__ explicit SweepSlots(const int tid) {
+        const int x = tid & (N - 1), r0 = tid >> LOGN;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) b[j] = opaque(Lay::at(r0 + R * j + O, x + O));
+    }
+    __device__ __forceinline__ int operator()(const int i) const { return b[i % PER] + R * (i - i % PER) * P; }
+};
 
 // KIND (dst_device.hpp): kDst for Dirichlet planes of (N-1)^2 points,
 // kRdftFwd / kRdftInv for periodic planes of N^2 points.
@@ -104,12 +125,22 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
     const auto dst = as_global(out(blockIdx.y) + plane_off);
     [[maybe_unused]] const auto bp = as_global(ACC == 2 ? base + blockIdx.y * base_bs + plane_off : base);
     if (done && done[blockIdx.y]) {
-        if constexpr (ACC == 2) plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) { dst[y * n + x] = (Tout)bp[y * n + x]; });
+        if constexpr (ACC == 2)
+            plane_sweep<LOGN, NT, n>(tid, [&](int, int y, int x) { dst[y * n + x] = (Tout)bp[y * n + x]; });
         return;
     }
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
-    const auto src = as_global(in(blockIdx.y) + plane_off);
+    // the plane is read through a buffer resource: per element a lane offset
+    // plus a compile-time one.  (The stores keep the natural global order and
+    // read the permuted LDS slots: sweeping the LDS slots in order and
+    // scattering rows on the global side issues fewer instructions but
+    // measured 5-9% slower.)
+    constexpr unsigned plane_pts = (unsigned)n * n;
+    const BufRange<Tin> src(in(blockIdx.y) + plane_off, plane_pts * sizeof(Tin));
+    const SweepSlots<LOGN, NT, C::P, S::N1, o> slot(tid);
+    constexpr int R = NT / N;
+    const int lane_in = (int)(((tid >> LOGN) * n + (tid & (N - 1))) * sizeof(Tin));
 
     DstLane<T, LOGN> L;
     L.init(tables, tid & (S::N1 - 1));
@@ -120,8 +151,8 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
         }
     }
     plane_stage<LOGN, NT, n, Tin>(
-        tid, [&](int y, int x) { return src[y * n + x]; },
-        [&](int y, int x, Tin v) { lds[LX::at(y + o, x + o)] = (T)v; });
+        tid, [&](int i, int, int) { return src.load(lane_in, (int)(R * i * n * sizeof(Tin))); },
+        [&](int i, int, int, Tin v) { lds[slot(i)] = (T)v; });
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, N / 2, LX, NT>(lds, L, sx, tid);   // pencils = rows    (x)
     __syncthreads();
@@ -129,10 +160,10 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
     __syncthreads();
     if constexpr (ACC == 2) {   // the FP32 base: every load in flight before the first store
         plane_stage<LOGN, NT, n, float>(
-            tid, [&](int y, int x) { return bp[y * n + x]; },
-            [&](int y, int x, float b) { dst[y * n + x] = (Tout)b + (Tout)lds[LX::at(IO::out(y), IO::out(x))]; });
+            tid, [&](int, int y, int x) { return bp[y * n + x]; },
+            [&](int, int y, int x, float b) { dst[y * n + x] = (Tout)b + (Tout)lds[LX::at(IO::out(y), IO::out(x))]; });
     } else {
-        plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) {
+        plane_sweep<LOGN, NT, n>(tid, [&](int, int y, int x) {
             const T v = lds[LX::at(IO::out(y), IO::out(x))];
             if constexpr (ACC == 1) dst[y * n + x] += (Tout)v;
             else dst[y * n + x] = (Tout)v;
@@ -225,7 +256,7 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<float>()))) vo
     const auto fp = as_global(f(p));
     const auto up = as_global(u(p));
     double rr = 0.0, ff = 0.0;
-    plane_sweep<LOGN, NT, n, 2>(tid, [&](int y, int x) {   // 8 loads per point: unroll little (registers)
+    plane_sweep<LOGN, NT, n, 2>(tid, [&](int, int y, int x) {   // 8 loads per point: unroll little (registers)
         double fv;
         const double r =
             residual_at<Tu>(fp, up, (long)z * n * n + y * n + x, x, y, z, n, n, nplanes, ix, iy, iz, fv);
@@ -240,7 +271,7 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<float>()))) vo
     dst_pairs<float, LOGN, N / 2, LY, NT>(lds, L, sy, tid);
     __syncthreads();
     float* dst = out + p * bs + (long)z * n * n;
-    plane_sweep<LOGN, NT, n>(tid, [&](int y, int x) { dst[y * n + x] = lds[LX::at(S::sigma(y + 1), S::sigma(x + 1))]; });
+    plane_sweep<LOGN, NT, n>(tid, [&](int, int y, int x) { dst[y * n + x] = lds[LX::at(S::sigma(y + 1), S::sigma(x + 1))]; });
 }
 
 template <typename Tin, typename Tout, typename T, int LOGN, int ACC, int KIND>
