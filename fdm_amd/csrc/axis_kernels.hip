@@ -65,6 +65,53 @@ __device__ __forceinline__ void staged_copy(const int tid, LD&& ld, ST&& st) {
 template <typename T>
 constexpr int kCopyBatch = sizeof(T) == 8 ? 32 : 16;
 
+// Element map of an x tile (TILE rows of N slots): thread tid owns columns
+// x = tid % XW + XW*c (c < CX) of rows r0 + R*i (i < RI), r0 = tid / XW,
+// XW = min(NT, N).  Global element (row, x) of the tile is at row*n + x, so
+// every element is a lane offset plus a compile-time one; the LDS rows of a
+// thread repeat their swizzle f() every PER elements (period N1 in the row).
+template <int LOGN, int NT, int TILE, int P, int N1>
+struct XTileMap {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int XW = NT < N ? NT : N, CX = N / XW, R = NT / XW, RI = R < TILE ? TILE / R : 1;
+    static constexpr int PER = N1 / gcd_int(R, N1);
+    static_assert((R < TILE ? RI * R == TILE : true) && CX * XW == N, "tile map");
+    using Lay = PlaneLayout<P, N1, true>;
+    int xb, r0;
+    bool on;   // the thread has rows (false only when R > TILE)
+    __device__ explicit XTileMap(const int tid) : xb(tid % XW), r0(tid / XW), on(R <= TILE || tid / XW < TILE) {}
+    // LDS row base of element i (column 0, f included): PER bases
+    __device__ int row_base(const int j) const { return Lay::at(r0 + R * j, 0); }
+};
+
+// LDS <-> global sweep of an x tile in batches of KB elements per thread: the
+// batch's loads are issued before its first store.  ld(c, i) / st(c, i, v)
+// get compile-time element indices; columns x >= NN are skipped, and so is
+// everything when `on` is false.
+template <int CX, int RI, int XW, int NN, int KB, typename V, typename LD, typename ST>
+__device__ __forceinline__ void xtile_sweep(const int xb, const bool on, LD&& ld, ST&& st) {
+    constexpr int E = CX * RI;
+    static_for<0, (E + KB - 1) / KB>([&](auto bb) {
+        constexpr int b0 = decltype(bb)::value * KB;
+        V v[KB];
+        static_for<0, KB>([&](auto kk) {
+            constexpr int e = decltype(kk)::value;
+            if constexpr (b0 + e < E) {
+                constexpr int c = (b0 + e) / RI, i = (b0 + e) % RI;
+                if (on && xb + XW * c < NN) v[e] = ld(std::integral_constant<int, c>{}, std::integral_constant<int, i>{});
+            }
+        });
+        __builtin_amdgcn_sched_barrier(0);
+        static_for<0, KB>([&](auto kk) {
+            constexpr int e = decltype(kk)::value;
+            if constexpr (b0 + e < E) {
+                constexpr int c = (b0 + e) / RI, i = (b0 + e) % RI;
+                if (on && xb + XW * c < NN) st(std::integral_constant<int, c>{}, std::integral_constant<int, i>{}, v[e]);
+            }
+        });
+    });
+}
+
 // rows = nrows pencils of IO::n points (N-1 Dirichlet, N periodic), contiguous;
 // batch via blockIdx.y.
 template <typename Tin, typename Tout, typename T, int LOGN, bool ACCUM, int KIND, int TILE_>
@@ -81,34 +128,43 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void xaxis_dst_kerne
     T* lds = reinterpret_cast<T*>(smem_raw);
     const int tid = threadIdx.x;
     const long r0 = (long)blockIdx.x * TILE;
-    const auto src = as_global(in(blockIdx.y) + r0 * n);
-    const auto dst = as_global(out(blockIdx.y) + r0 * n);
     const int rows_here = (int)((nrows - r0) < TILE ? (nrows - r0) : TILE);
+    // the tile's rows through buffer resources sized to the rows present:
+    // rows past the last one read 0 and their stores are dropped
+    const unsigned pts = (unsigned)rows_here * n;
+    const BufRange<Tin> src(in(blockIdx.y) + r0 * n, pts * sizeof(Tin));
+    const BufRange<Tout> dst(out(blockIdx.y) + r0 * n, pts * sizeof(Tout));
+    using M = XTileMap<LOGN, NT, TILE, C::P, S::N1>;
+    const M m(tid);
+    int rb[M::PER];
+#pragma unroll
+    for (int j = 0; j < M::PER; ++j) rb[j] = opaque(m.row_base(j));
+    // LDS slot of element (c, i) at column col: rows R*(i - i % PER) apart
+    // share the swizzle
+    auto lslot = [&](int i, int col) { return rb[i % M::PER] + M::R * (i - i % M::PER) * C::P + col; };
+    const int lane_in = (m.r0 * n + m.xb) * (int)sizeof(Tin), lane_out = (m.r0 * n + m.xb) * (int)sizeof(Tout);
 
     DstLane<T, LOGN> L;
     L.init(tables, tid & (S::N1 - 1));
     if constexpr (KIND == kDst)
         for (int p = tid; p < TILE; p += NT) lds[LX::at(p, 0)] = T(0);
-    staged_copy<TILE * n, NT, kCopyBatch<T>, Tin>(
-        tid,
-        [&](int idx) {
-            const int p = idx / n, x = idx - p * n;
-            const Tin raw = src[(p < rows_here ? p : rows_here - 1) * n + x];
-            return p < rows_here ? raw : Tin(0);
-        },
-        [&](int idx, Tin v) {
-            const int p = idx / n, x = idx - p * n;
-            lds[LX::at(p, x + o)] = (T)v;
-        });
+    xtile_sweep<M::CX, M::RI, M::XW, n, kCopyBatch<T>, Tin>(
+        m.xb, m.on,
+        [&](auto c, auto i) { return src.load(lane_in, (int)((M::XW * c + M::R * i * n) * sizeof(Tin))); },
+        [&](auto c, auto i, Tin v) { lds[lslot(i, m.xb + M::XW * c + o)] = (T)v; });
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, TILE / 2, LX, NT>(lds, L, scale, tid);
     __syncthreads();
-    for (int idx = tid; idx < rows_here * n; idx += NT) {
-        const int p = idx / n, x = idx - p * n;
-        const T v = lds[LX::at(p, IO::out(x))];
-        if constexpr (ACCUM) dst[idx] += (Tout)v;
-        else dst[idx] = (Tout)v;
-    }
+    int oc[M::CX];
+#pragma unroll
+    for (int c = 0; c < M::CX; ++c) oc[c] = IO::out(m.xb + M::XW * c < n ? m.xb + M::XW * c : 0);
+    xtile_sweep<M::CX, M::RI, M::XW, n, 2 * kCopyBatch<T>, T>(
+        m.xb, m.on, [&](auto c, auto i) { return lds[lslot(i, oc[c])]; },
+        [&](auto c, auto i, T v) {
+            const int so = (int)((M::XW * c + M::R * i * n) * sizeof(Tout));
+            if constexpr (ACCUM) dst.store(dst.load(lane_out, so) + (Tout)v, lane_out, so);
+            else dst.store((Tout)v, lane_out, so);
+        });
 }
 
 // Columns of plane blockIdx.y (nplanes planes of [IO::n][nx]); batch via

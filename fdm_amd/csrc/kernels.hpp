@@ -46,6 +46,8 @@ __device__ __forceinline__ global_ptr<T> as_global(T* p) {
 // instruction (a 64-bit global address costs two per 4 KB window).  Accesses
 // at offsets >= `bytes` read 0 and are dropped on store.  The third word
 // 0x00020000 is the raw 32-bit data format of the gfx9 family (gfx950).
+constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 template <typename T>

@@ -83,8 +83,6 @@ __device__ __forceinline__ void plane_stage(const int tid, LD&& ld, ST&& st) {
     }
 }
 
-constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
-
 // LDS slots of a sweep's elements: element i of thread tid is slot
 // (r0 + R*i + o, x + o) of a PlaneLayout (x = tid % N, r0 = tid / N,
 // R = NT / N).  The row swizzle f() has period N1 in the row, so rows a
