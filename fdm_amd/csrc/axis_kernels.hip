@@ -185,31 +185,54 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void yaxis_dst_kerne
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TILE;
     const long poff = (long)blockIdx.y * n * nx;
-    const auto src = as_global(in(blockIdx.z) + poff);
-    const auto dst = as_global(out(blockIdx.z) + poff);
+    // thread tid owns tile column p = tid % TILE (plane column x0 + p) and
+    // rows y0 + YR*i, y0 = tid / TILE: the LDS slot of a row is the pencil's
+    // base plus a compile-time offset, the global one a lane offset plus a
+    // wave-uniform one (buffer resources over the plane)
+    static_assert(NT % TILE == 0, "whole tile rows per pass");
+    constexpr int YR = NT / TILE, YI = (n + YR - 1) / YR;
+    const unsigned pts = (unsigned)n * nx;
+    const BufRange<Tin> src(in(blockIdx.z) + poff, pts * sizeof(Tin));
+    const BufRange<Tout> dst(out(blockIdx.z) + poff, pts * sizeof(Tout));
+    const int p = tid % TILE, y0 = tid / TILE;
+    const bool col_ok = x0 + p < nx;
+    const int lane_in = (y0 * nx + (col_ok ? x0 + p : 0)) * (int)sizeof(Tin);
+    const int lane_out = (y0 * nx + (col_ok ? x0 + p : 0)) * (int)sizeof(Tout);
+    const int pb = opaque(LT::at(p, 0));
 
     DstLane<T, LOGN> L;
     L.init(tables, tid & (S::N1 - 1));
     if constexpr (KIND == kDst)
-        for (int p = tid; p < TILE; p += NT) lds[LT::at(p, 0)] = T(0);   // slot 0 of every pencil
-    staged_copy<n * TILE, NT, kCopyBatch<T>, Tin>(
-        tid,
-        [&](int idx) {
-            const int y = idx / TILE, x = x0 + (idx - y * TILE);
-            const Tin raw = src[(long)(y < n ? y : n - 1) * nx + (x < nx ? x : nx - 1)];
-            return x < nx ? raw : Tin(0);
-        },
-        [&](int idx, Tin v) {
-            const int y = idx / TILE, p = idx - y * TILE;
-            lds[LT::at(p, y + o)] = (T)v;
+        for (int q = tid; q < TILE; q += NT) lds[LT::at(q, 0)] = T(0);   // slot 0 of every pencil
+    constexpr int KB = kCopyBatch<T>;
+    static_for<0, (YI + KB - 1) / KB>([&](auto bb) {
+        constexpr int b0 = decltype(bb)::value * KB;
+        Tin v[KB];
+        static_for<0, KB>([&](auto kk) {
+            constexpr int i = b0 + decltype(kk)::value;
+            if constexpr (i < YI)
+                if (YR * i + YR <= n || y0 + YR * i < n) {
+                    const Tin raw = src.load(lane_in, YR * i * nx * (int)sizeof(Tin));
+                    v[i - b0] = col_ok ? raw : Tin(0);
+                }
         });
+        __builtin_amdgcn_sched_barrier(0);
+        static_for<0, KB>([&](auto kk) {
+            constexpr int i = b0 + decltype(kk)::value;
+            if constexpr (i < YI)
+                if (YR * i + YR <= n || y0 + YR * i < n) lds[pb + y0 + YR * i + o] = (T)v[i - b0];
+        });
+    });
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, TILE / 2, LT, NT>(lds, L, scale, tid);
     __syncthreads();
-    for (int idx = tid; idx < n * TILE; idx += NT) {
-        const int y = idx / TILE, p = idx - y * TILE;
-        const int x = x0 + p;
-        if (x < nx) dst[(long)y * nx + x] = (Tout)lds[LT::at(p, IO::out(y))];
+    if (col_ok) {
+#pragma unroll 16
+        for (int i = 0; i < YI; ++i) {
+            const int y = y0 + YR * i;
+            if (YR * i + YR <= n || y < n)
+                dst.store((Tout)lds[pb + IO::out(y)], lane_out, YR * i * nx * (int)sizeof(Tout));
+        }
     }
 }
 
