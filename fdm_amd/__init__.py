@@ -34,7 +34,7 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdm
            "fdmgpu_rect_set_scales", "fdmgpu_set_refine",
            "fdmgpu_solve_host", "fdmgpu_solve_device", "fdmgpu_solve_batch_device", "fdmgpu_solve_table_device",
            "fdmgpu_plan_variants", "fdmgpu_plan_set_variant", "fdmgpu_plan_get_variant", "fdmgpu_get_stats",
-           "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_group_create",
+           "fdmgpu_plan_footprint", "fdmgpu_set_timing", "fdmgpu_get_timing", "fdmgpu_set_timing_kernels", "fdmgpu_group_create",
            "fdmgpu_group_solve", "fdmgpu_group_result", "fdmgpu_group_destroy", "fdmgpu_server_create",
            "fdmgpu_server_submit", "fdmgpu_server_set_refine", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
            "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_tune_set", "fdmgpu_ns_create", "fdmgpu_ns_step", "fdmgpu_ns_get", "fdmgpu_ns_set", "fdmgpu_ns_info",
@@ -114,6 +114,7 @@ def _lib_handle():
         L.fdmgpu_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.fdmgpu_plan_footprint.argtypes = [vp, i, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
         L.fdmgpu_set_timing.argtypes = [vp, i]
+        L.fdmgpu_set_timing_kernels.argtypes = [vp, C.c_uint]
         L.fdmgpu_get_timing.argtypes = [vp, i, C.POINTER(d), C.POINTER(i)]
         ull = C.c_ulonglong
         L.fdmgpu_solve_table_device.argtypes = [vp, i, vp, vp, vp]
@@ -249,9 +250,16 @@ class _Plan:
         _check(_lib_handle().fdmgpu_solve_batch_device(self._h, count, pa, pr, st), f"{self._kind}.solve_batch")
         return ans
 
-    def set_timing(self, enable: bool = True):
-        """Enable (and reset) per-kernel HIP-event timing."""
-        _check(_lib_handle().fdmgpu_set_timing(self._h, int(enable)), "set_timing")
+    def set_timing(self, enable: bool = True, kernels=None):
+        """Enable (and reset) per-kernel HIP-event timing: of every kernel
+        class, or only of the classes named in `kernels` (KERNEL_NAMES)."""
+        if kernels is None:
+            _check(_lib_handle().fdmgpu_set_timing(self._h, int(enable)), "set_timing")
+            return
+        mask = 0
+        for name in kernels:
+            mask |= 1 << KERNEL_NAMES.index(name)
+        _check(_lib_handle().fdmgpu_set_timing_kernels(self._h, mask if enable else 0), "set_timing")
 
     def timing(self):
         """{kernel name: (total ms, launches)} since the last set_timing (synchronises)."""

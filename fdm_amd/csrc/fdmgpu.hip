@@ -267,7 +267,7 @@ struct Timed {
     size_t slot = 0;
     hipStream_t s;
     bool on;
-    Timed(fdmgpu_plan_t* P_, int k, hipStream_t s_) : P(P_), s(s_), on(P_->timing) {
+    Timed(fdmgpu_plan_t* P_, int k, hipStream_t s_) : P(P_), s(s_), on((P_->timing >> k) & 1u) {
         if (!on) return;
         slot = P->nrec++;
         while (P->evs.size() < 2 * P->nrec) {
@@ -1070,17 +1070,21 @@ int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resid
     return FDMGPU_OK;
 }
 
-int fdmgpu_set_timing(fdmgpu_plan plan, int enable) {
+int fdmgpu_set_timing_kernels(fdmgpu_plan plan, unsigned mask) {
     if (!plan) return FDMGPU_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(plan->device));
     int rc = harvest_timing(plan);
     if (rc) return rc;
-    plan->timing = enable != 0;
+    plan->timing = mask & ((1u << FDMGPU_NUM_KERNELS) - 1);
     for (int k = 0; k < FDMGPU_NUM_KERNELS; k++) {
         plan->tot_ms[k] = 0;
         plan->launches[k] = 0;
     }
     return FDMGPU_OK;
+}
+
+int fdmgpu_set_timing(fdmgpu_plan plan, int enable) {
+    return fdmgpu_set_timing_kernels(plan, enable ? ~0u : 0u);
 }
 
 int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launches) {

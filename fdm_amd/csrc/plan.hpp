@@ -82,8 +82,9 @@ struct fdmgpu_plan_t {
     hipStream_t own = nullptr;
     hipStream_t last_stream = nullptr;
     int last_count = 0;
-    // per-kernel timing (fdmgpu_set_timing)
-    bool timing = false;
+    // per-kernel timing: bit k set = launches of kernel class k are timed
+    // (fdmgpu_set_timing, fdmgpu_set_timing_kernels)
+    unsigned timing = 0;
     std::vector<hipEvent_t> evs;        // 2 per recorded launch
     std::vector<int> rec_kernel;
     size_t nrec = 0;

@@ -183,6 +183,10 @@ int fdmgpu_plan_footprint(fdmgpu_plan plan, int count, unsigned long long* resid
 #define FDMGPU_NUM_KERNELS 12
 int fdmgpu_set_timing(fdmgpu_plan plan, int enable);
 int fdmgpu_get_timing(fdmgpu_plan plan, int kernel, double* total_ms, int* launches);
+/* The same with events around the launches of the kernel classes in `mask`
+ * only (bit k = class k): a measurement of one class's launches that leaves
+ * the others uninstrumented.  fdmgpu_set_timing(plan, e) is mask ~0 or 0. */
+int fdmgpu_set_timing_kernels(fdmgpu_plan plan, unsigned mask);
 
 /* ---- heterogeneous batches and serving ------------------------------------
  * No reference counterpart: the reference solves one problem per call

@@ -215,6 +215,37 @@ def test_cube_device_batch_matches_single(fdm):
         assert rel2(batch[1], ref) <= (1e-12 if prec == "f64" else 1e-9)
 
 
+def test_timing_kernel_mask(fdm):
+    """fdmgpu_set_timing_kernels: events around the named classes' launches
+    only; the timed launches match the all-classes count for that class, and
+    the results are the same with timing on or off (bitwise)."""
+    from fdm_amd import hip
+    say("timing mask, refine 31^3 batch of 2")
+    n = 31
+    h, l = O.cube_geometry(n)
+    fs = np.stack([O.manufactured_cube(n, seed=s)[1] for s in (21, 22)])
+    s = fdm.LaplCube(h, h, h, l, l, l, n, n, n, precision="refine")
+    s.set_refine(1e-10, 2)
+    rhs = hip.DeviceArray.from_host(fs)
+    out = hip.DeviceArray(fs.shape, np.float64)
+    s.solve_batch(out, rhs)
+    hip.synchronize()
+    plain = out.download()
+    s.set_timing(True)
+    s.solve_batch(out, rhs)
+    full = s.timing()
+    assert np.array_equal(out.download(), plain)
+    assert "residual" in full and len(full) >= 3
+    s.set_timing(True, kernels=["residual"])
+    s.solve_batch(out, rhs)
+    only = s.timing()
+    assert list(only) == ["residual"] and only["residual"][1] == full["residual"][1]
+    assert np.array_equal(out.download(), plain)
+    s.set_timing(False)
+    s.solve_batch(out, rhs)
+    assert s.timing() == {}
+
+
 def test_torch_interop_subprocess():
     # torch brings its own HIP runtime; with torch initialised first, libfdmgpu.so binds
     # to it and accepts torch tensors.  Run in a fresh process.
