@@ -86,30 +86,25 @@ struct XTileMap {
 
 // LDS <-> global sweep of an x tile in batches of KB elements per thread: the
 // batch's loads are issued before its first store.  ld(c, i) / st(c, i, v)
-// get compile-time element indices; columns x >= NN are skipped, and so is
-// everything when `on` is false.
+// get element indices that are constants once the loops are unrolled;
+// columns x >= NN are skipped, and so is everything when `on` is false.
 template <int CX, int RI, int XW, int NN, int KB, typename V, typename LD, typename ST>
 __device__ __forceinline__ void xtile_sweep(const int xb, const bool on, LD&& ld, ST&& st) {
-    constexpr int E = CX * RI;
-    static_for<0, (E + KB - 1) / KB>([&](auto bb) {
-        constexpr int b0 = decltype(bb)::value * KB;
-        V v[KB];
-        static_for<0, KB>([&](auto kk) {
-            constexpr int e = decltype(kk)::value;
-            if constexpr (b0 + e < E) {
-                constexpr int c = (b0 + e) / RI, i = (b0 + e) % RI;
-                if (on && xb + XW * c < NN) v[e] = ld(std::integral_constant<int, c>{}, std::integral_constant<int, i>{});
-            }
-        });
-        __builtin_amdgcn_sched_barrier(0);
-        static_for<0, KB>([&](auto kk) {
-            constexpr int e = decltype(kk)::value;
-            if constexpr (b0 + e < E) {
-                constexpr int c = (b0 + e) / RI, i = (b0 + e) % RI;
-                if (on && xb + XW * c < NN) st(std::integral_constant<int, c>{}, std::integral_constant<int, i>{}, v[e]);
-            }
-        });
-    });
+#pragma unroll
+    for (int c = 0; c < CX; ++c) {
+        if (!on || xb + XW * c >= NN) continue;   // one branch per column, none per element
+#pragma unroll
+        for (int b0 = 0; b0 < RI; b0 += KB) {
+            V v[KB];
+#pragma unroll
+            for (int e = 0; e < KB; ++e)
+                if (b0 + e < RI) v[e] = ld(c, b0 + e);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int e = 0; e < KB; ++e)
+                if (b0 + e < RI) st(c, b0 + e, v[e]);
+        }
+    }
 }
 
 // rows = nrows pencils of IO::n points (N-1 Dirichlet, N periodic), contiguous;
@@ -150,8 +145,8 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void xaxis_dst_kerne
         for (int p = tid; p < TILE; p += NT) lds[LX::at(p, 0)] = T(0);
     xtile_sweep<M::CX, M::RI, M::XW, n, kCopyBatch<T>, Tin>(
         m.xb, m.on,
-        [&](auto c, auto i) { return src.load(lane_in, (int)((M::XW * c + M::R * i * n) * sizeof(Tin))); },
-        [&](auto c, auto i, Tin v) { lds[lslot(i, m.xb + M::XW * c + o)] = (T)v; });
+        [&](int c, int i) { return src.load(lane_in, (int)((M::XW * c + M::R * i * n) * sizeof(Tin))); },
+        [&](int c, int i, Tin v) { lds[lslot(i, m.xb + M::XW * c + o)] = (T)v; });
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, TILE / 2, LX, NT>(lds, L, scale, tid);
     __syncthreads();
@@ -159,8 +154,8 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void xaxis_dst_kerne
 #pragma unroll
     for (int c = 0; c < M::CX; ++c) oc[c] = IO::out(m.xb + M::XW * c < n ? m.xb + M::XW * c : 0);
     xtile_sweep<M::CX, M::RI, M::XW, n, 2 * kCopyBatch<T>, T>(
-        m.xb, m.on, [&](auto c, auto i) { return lds[lslot(i, oc[c])]; },
-        [&](auto c, auto i, T v) {
+        m.xb, m.on, [&](int c, int i) { return lds[lslot(i, oc[c])]; },
+        [&](int c, int i, T v) {
             const int so = (int)((M::XW * c + M::R * i * n) * sizeof(Tout));
             if constexpr (ACCUM) dst.store(dst.load(lane_out, so) + (Tout)v, lane_out, so);
             else dst.store((Tout)v, lane_out, so);
@@ -205,24 +200,24 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void yaxis_dst_kerne
     if constexpr (KIND == kDst)
         for (int q = tid; q < TILE; q += NT) lds[LT::at(q, 0)] = T(0);   // slot 0 of every pencil
     constexpr int KB = kCopyBatch<T>;
-    static_for<0, (YI + KB - 1) / KB>([&](auto bb) {
-        constexpr int b0 = decltype(bb)::value * KB;
+#pragma unroll
+    for (int b0 = 0; b0 < YI; b0 += KB) {
         Tin v[KB];
-        static_for<0, KB>([&](auto kk) {
-            constexpr int i = b0 + decltype(kk)::value;
-            if constexpr (i < YI)
-                if (YR * i + YR <= n || y0 + YR * i < n) {
-                    const Tin raw = src.load(lane_in, YR * i * nx * (int)sizeof(Tin));
-                    v[i - b0] = col_ok ? raw : Tin(0);
-                }
-        });
+#pragma unroll
+        for (int e = 0; e < KB; ++e) {
+            const int i = b0 + e;
+            if (i < YI && (YR * i + YR <= n || y0 + YR * i < n)) {
+                const Tin raw = src.load(lane_in, YR * i * nx * (int)sizeof(Tin));
+                v[e] = col_ok ? raw : Tin(0);
+            }
+        }
         __builtin_amdgcn_sched_barrier(0);
-        static_for<0, KB>([&](auto kk) {
-            constexpr int i = b0 + decltype(kk)::value;
-            if constexpr (i < YI)
-                if (YR * i + YR <= n || y0 + YR * i < n) lds[pb + y0 + YR * i + o] = (T)v[i - b0];
-        });
-    });
+#pragma unroll
+        for (int e = 0; e < KB; ++e) {
+            const int i = b0 + e;
+            if (i < YI && (YR * i + YR <= n || y0 + YR * i < n)) lds[pb + y0 + YR * i + o] = (T)v[e];
+        }
+    }
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, TILE / 2, LT, NT>(lds, L, scale, tid);
     __syncthreads();
