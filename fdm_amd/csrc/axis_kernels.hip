@@ -302,6 +302,9 @@ template <typename Tin, typename Tout, typename T, int LOGN, int KIND, int TILE>
 static hipError_t yaxis_k(const AxisArgs& a) {
     using C = TileCfg<LOGN, T, TILE>;
     auto k = yaxis_dst_kernel<Tin, Tout, T, LOGN, KIND, TILE>;
+    // the kernel addresses a plane through 32-bit byte offsets
+    constexpr size_t elem = sizeof(Tin) > sizeof(Tout) ? sizeof(Tin) : sizeof(Tout);
+    if (((size_t)1 << LOGN) * (size_t)a.nx * elem >= ((size_t)1 << 31)) return hipErrorInvalidValue;
     const size_t lds = C::lds();
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
