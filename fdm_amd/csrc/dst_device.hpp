@@ -459,6 +459,149 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, 
     pencil_pairs<kDst, T, LOGN, JOBS, Lay, NT>(lds, L, scale, tid);
 }
 
+// The DST of the columns of a 128-wide plane (pencils = columns of a
+// PlaneLayout<P, 8, false>; N = 128, N1 = 8, N2 = 16, KC = 8), the same
+// transform as pencil_pairs<kDst> with cheaper addressing.  Every slot a job's
+// lane t touches is 8m + (t ^ c), c a compile-time constant and m either a
+// constant or t, 2t or 15 - t; the row swizzle f() has period 8, so the byte
+// offset is (a per-lane base for m) + H[c] + 8 rows * (constant part of m),
+// H[c] = offset of row t ^ c: one add per distinct (base, c) where the
+// generic path multiplies and swizzles per access.
+template <typename T, int P, int JOBS, int NT>
+__device__ __forceinline__ void dst_cols128(T* __restrict__ lds, const DstLane<T, 7>& L, const T scale,
+                                            const int tid) {
+    using S = DstShape<7>;
+    constexpr int N = 128, N1 = 8, N2 = 16, KC = 8;
+    static_assert(S::N1 == N1 && S::N2 == N2 && S::KC == KC, "N = 128 shape");
+    static_assert(JOBS * N1 == NT, "one job per thread");
+    using Sw = RowSwizzle<N1>;
+    constexpr int ES = (int)sizeof(T);
+    constexpr int ROW8 = 8 * P * ES;   // bytes of 8 rows
+    constexpr int PBO = JOBS * ES;     // pencil g + JOBS
+    const int t = tid & (N1 - 1);
+    const int g = tid / N1;
+    char* const base = reinterpret_cast<char*>(lds);
+    auto rowb = [](int r) { return (r * P + Sw::f(r)) * ES; };
+    int H[N1];
+#pragma unroll
+    for (int c = 0; c < N1; ++c) H[c] = opaque(rowb(t ^ c));
+    const int pb = g * ES;
+    const int b0 = opaque(pb), b1 = opaque(pb + ROW8 * t), b2 = opaque(pb + 2 * ROW8 * t),
+              b3 = opaque(pb + ROW8 * (15 - t));
+    // the slots outside the 8m + (t ^ c) form: N - t (step 1, j = t) and the
+    // mirror of k = 8t (step 3)
+    const int rA = opaque(pb + rowb((N - t) & (N - 1)));
+    const int rB = opaque(pb + rowb(8 - t));   // f(8) = f(0): row 8(15 - j2) + 8 - t for every t
+    const int rC = opaque(pb + rowb(S::posZ((N - 8 * t) & (N - 1))));
+    auto A = [&](int off) -> T& { return *reinterpret_cast<T*>(base + off); };
+    auto B = [&](int off) -> T& { return *reinterpret_cast<T*>(base + off + PBO); };
+
+    // ---- step 1: pre-processing + 16-point DFTs over j2 (j = t + 8 j2)
+    T yr[N2], yi[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+        const int o0 = b0 + H[0] + ROW8 * j2;
+        const int o1 = j2 == 0 ? rA : rB + ROW8 * (15 - j2);
+        const T a0 = A(o0), a1 = A(o1), c0 = B(o0), c1 = B(o1);
+        const T sp = L.sp[j2];
+        yr[j2] = fma(sp, a0 + a1, -a1);
+        yi[j2] = fma(sp, c0 + c1, -c1);
+    }
+    dft<N2>(yr, yi);
+#pragma unroll
+    for (int k2 = 1; k2 < N2; ++k2) {
+        const T c = L.twr[k2], s = L.twi[k2];
+        const T r = yr[k2] * c - yi[k2] * s;
+        const T i = yr[k2] * s + yi[k2] * c;
+        yr[k2] = r;
+        yi[k2] = i;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {   // slot 8 k2 + (t ^ (k2 mod 8))
+        const int o = b0 + H[k2 & 7] + ROW8 * k2;
+        A(o) = yr[k2];
+        B(o) = yi[k2];
+    }
+    wave_lds_sync();
+
+    // ---- step 2: 8-point DFTs over j1 for k2 = t + 8 r (slot 8t + (j1 ^ t) + 64 r)
+    T zr[N2], zi[N2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        T vr[N1], vi[N1];
+#pragma unroll
+        for (int j1 = 0; j1 < N1; ++j1) {
+            const int o = b1 + H[j1] + ROW8 * 8 * r;
+            vr[j1] = A(o);
+            vi[j1] = B(o);
+        }
+        dft<N1>(vr, vi);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+            zr[r * N1 + k1] = vr[k1];
+            zi[r * N1 + k1] = vi[k1];
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {   // posZ(t + cz) = cz + (t ^ ((r + 2 k1) mod 8)), cz = 8 (r + 2 k1)
+            const int m = r + 2 * k1;
+            const int o = b0 + H[m & 7] + ROW8 * m;
+            A(o) = zr[r * N1 + k1];
+            B(o) = zi[r * N1 + k1];
+        }
+    }
+    wave_lds_sync();
+
+    // ---- step 3: even outputs directly, odd outputs by prefix sum (as pencil_pairs)
+    const T hs = T(0.5) * scale;
+    T ae[KC], ao[KC], be[KC], bo[KC];
+    T runA = T(0), runB = T(0);
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+        // posZ(8t + i) = 8t + (i ^ t); posZ(N - 8t - i) = 8 (15 - t) + (t ^ (7 ^ (8 - i))) for i > 0
+        const int ok = b1 + H[i];
+        const int on = i == 0 ? rC : b3 + H[7 ^ (8 - i)];
+        const T zkr = A(ok), zki = B(ok);
+        const T znr = A(on), zni = B(on);
+        ae[i] = zni - zki;
+        be[i] = zkr - znr;
+        T dA = zkr + znr;
+        T dB = zki + zni;
+        if (i == 0 && t == 0) {
+            dA *= T(0.5);
+            dB *= T(0.5);
+        }
+        runA += dA;
+        runB += dB;
+        ao[i] = runA;
+        bo[i] = runB;
+    }
+    const T offA = lane_scan_incl<N1>(runA) - runA;
+    const T offB = lane_scan_incl<N1>(runB) - runB;
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+        ao[i] += offA;
+        bo[i] += offB;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+        // sigma(2k + e) for k = 8t + i: 16t + ((2i + e) mod 8 ^ t) + 8 ((2i + e) div 8)
+        const bool k0 = i == 0 && t == 0;
+        const int me = 2 * i, mo = 2 * i + 1;
+        const int oe = b2 + H[me & 7] + ROW8 * (me >> 3), oo = b2 + H[mo & 7] + ROW8 * (mo >> 3);
+        A(oe) = k0 ? T(0) : ae[i] * hs;
+        B(oe) = k0 ? T(0) : be[i] * hs;
+        A(oo) = ao[i] * hs;
+        B(oo) = bo[i] * hs;
+    }
+    wave_lds_sync();
+}
+
 // ---------------------------------------------------------------------------
 // Constant-coefficient cyclic reduction along z for one (kx,ky) mode.
 //

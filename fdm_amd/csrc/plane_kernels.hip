@@ -154,7 +154,10 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, N / 2, LX, NT>(lds, L, sx, tid);   // pencils = rows    (x)
     __syncthreads();
-    pencil_pairs<KIND, T, LOGN, N / 2, LY, NT>(lds, L, sy, tid);   // pencils = columns (y)
+    if constexpr (KIND == kDst && LOGN == 7)
+        dst_cols128<T, C::P, N / 2, NT>(lds, L, sy, tid);   // pencils = columns (y)
+    else
+        pencil_pairs<KIND, T, LOGN, N / 2, LY, NT>(lds, L, sy, tid);
     __syncthreads();
     if constexpr (ACC == 2) {   // the FP32 base: every load in flight before the first store
         plane_stage<LOGN, NT, n, float>(
