@@ -459,33 +459,35 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, 
     pencil_pairs<kDst, T, LOGN, JOBS, Lay, NT>(lds, L, scale, tid);
 }
 
-// The DST of the columns of a 128-wide plane (pencils = columns of a
-// PlaneLayout<P, 8, false>; N = 128, N1 = 8, N2 = 16, KC = 8), the same
-// transform as pencil_pairs<kDst> with cheaper addressing.  Every slot a job's
-// lane t touches is 8m + (t ^ c), c a compile-time constant and m either a
-// constant or t, 2t or 15 - t; the row swizzle f() has period 8, so the byte
-// offset is (a per-lane base for m) + H[c] + 8 rows * (constant part of m),
-// H[c] = offset of row t ^ c: one add per distinct (base, c) where the
-// generic path multiplies and swizzles per access.
-template <typename T, int P, int JOBS, int NT>
-__device__ __forceinline__ void dst_cols128(T* __restrict__ lds, const DstLane<T, 7>& L, const T scale,
-                                            const int tid) {
+// The DST of the rows (COLS false: PlaneLayout<P, 8, true>) or the columns
+// (COLS true: PlaneLayout<P, 8, false>) of a 128-wide plane (N = 128, N1 = 8,
+// N2 = 16, KC = 8), the same transform as pencil_pairs<kDst> with cheaper
+// addressing.  Every slot a job's lane t touches is 8m + (t ^ c), c a
+// compile-time constant and m either a constant or t, 2t or 15 - t; the row
+// swizzle f() has period 8, so the byte offset is (a per-lane base for m) +
+// H[c] + (8 slots) * (constant part of m), H[c] = offset of slot t ^ c: one
+// add per distinct (base, c) where the generic path computes the slot and
+// its swizzle per access.
+template <typename T, int P, int JOBS, int NT, bool COLS>
+__device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const DstLane<T, 7>& L, const T scale,
+                                             const int tid) {
     using S = DstShape<7>;
     constexpr int N = 128, N1 = 8, N2 = 16, KC = 8;
     static_assert(S::N1 == N1 && S::N2 == N2 && S::KC == KC, "N = 128 shape");
     static_assert(JOBS * N1 == NT, "one job per thread");
     using Sw = RowSwizzle<N1>;
     constexpr int ES = (int)sizeof(T);
-    constexpr int ROW8 = 8 * P * ES;   // bytes of 8 rows
-    constexpr int PBO = JOBS * ES;     // pencil g + JOBS
+    constexpr int ROW8 = COLS ? 8 * P * ES : 8 * ES;   // bytes of 8 slots
+    constexpr int PBO = COLS ? JOBS * ES : JOBS * P * ES;   // pencil g + JOBS (f(g + JOBS) = f(g))
+    static_assert(JOBS % N1 == 0, "swizzle period");
     const int t = tid & (N1 - 1);
     const int g = tid / N1;
     char* const base = reinterpret_cast<char*>(lds);
-    auto rowb = [](int r) { return (r * P + Sw::f(r)) * ES; };
+    auto rowb = [](int r) { return COLS ? (r * P + Sw::f(r)) * ES : r * ES; };   // slot r of pencil 0
     int H[N1];
 #pragma unroll
     for (int c = 0; c < N1; ++c) H[c] = opaque(rowb(t ^ c));
-    const int pb = g * ES;
+    const int pb = COLS ? g * ES : (g * P + Sw::f(g)) * ES;
     const int b0 = opaque(pb), b1 = opaque(pb + ROW8 * t), b2 = opaque(pb + 2 * ROW8 * t),
               b3 = opaque(pb + ROW8 * (15 - t));
     // the slots outside the 8m + (t ^ c) form: N - t (step 1, j = t) and the

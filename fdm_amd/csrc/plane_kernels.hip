@@ -154,8 +154,10 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
     __syncthreads();
     pencil_pairs<KIND, T, LOGN, N / 2, LX, NT>(lds, L, sx, tid);   // pencils = rows    (x)
     __syncthreads();
+    // the y pass at N = 128 with per-lane slot tables (the x pass measured
+    // 1-3% slower with them: its generic addressing pairs LDS accesses)
     if constexpr (KIND == kDst && LOGN == 7)
-        dst_cols128<T, C::P, N / 2, NT>(lds, L, sy, tid);   // pencils = columns (y)
+        dst_pairs128<T, C::P, N / 2, NT, true>(lds, L, sy, tid);   // pencils = columns (y)
     else
         pencil_pairs<KIND, T, LOGN, N / 2, LY, NT>(lds, L, sy, tid);
     __syncthreads();
