@@ -153,13 +153,31 @@ __global__ __launch_bounds__((TileCfg<LOGN, T, TILE_>::NT)) void xaxis_dst_kerne
     int oc[M::CX];
 #pragma unroll
     for (int c = 0; c < M::CX; ++c) oc[c] = IO::out(m.xb + M::XW * c < n ? m.xb + M::XW * c : 0);
-    xtile_sweep<M::CX, M::RI, M::XW, n, 2 * kCopyBatch<T>, T>(
-        m.xb, m.on, [&](int c, int i) { return lds[lslot(i, oc[c])]; },
-        [&](int c, int i, T v) {
-            const int so = (int)((M::XW * c + M::R * i * n) * sizeof(Tout));
-            if constexpr (ACCUM) dst.store(dst.load(lane_out, so) + (Tout)v, lane_out, so);
-            else dst.store((Tout)v, lane_out, so);
-        });
+    if constexpr (ACCUM) {
+        // out += result: a batch's global loads are all in flight before its
+        // first add and store
+        struct Acc {
+            Tout old;
+            T v;
+        };
+        xtile_sweep<M::CX, M::RI, M::XW, n, kCopyBatch<T>, Acc>(
+            m.xb, m.on,
+            [&](int c, int i) {
+                const int so = (int)((M::XW * c + M::R * i * n) * sizeof(Tout));
+                return Acc{dst.load(lane_out, so), lds[lslot(i, oc[c])]};
+            },
+            [&](int c, int i, Acc a) {
+                const int so = (int)((M::XW * c + M::R * i * n) * sizeof(Tout));
+                dst.store(a.old + (Tout)a.v, lane_out, so);
+            });
+    } else {
+        xtile_sweep<M::CX, M::RI, M::XW, n, 2 * kCopyBatch<T>, T>(
+            m.xb, m.on, [&](int c, int i) { return lds[lslot(i, oc[c])]; },
+            [&](int c, int i, T v) {
+                const int so = (int)((M::XW * c + M::R * i * n) * sizeof(Tout));
+                dst.store((Tout)v, lane_out, so);
+            });
+    }
 }
 
 // Columns of plane blockIdx.y (nplanes planes of [IO::n][nx]); batch via
