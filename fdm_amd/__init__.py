@@ -241,12 +241,21 @@ class _Plan:
         count = ans.shape[0]
         if tuple(ans.shape[1:]) != self.shape or tuple(rhs.shape) != tuple(ans.shape):
             raise ValueError("batch shape mismatch")
-        if _is_dev(ans):
+        if _is_dev(ans) and _is_dev(rhs):
+            for a, nm in ((ans, "ans"), (rhs, "rhs")):
+                if a.dtype != self.dtype:
+                    raise TypeError(f"{nm} must be {np.dtype(self.dtype).name}")
             pa, pr, st = ans.ptr, rhs.ptr, _stream_handle(stream)
-        else:
+        elif _is_torch(ans) and _is_torch(rhs):
             import torch
+            want = torch.float32 if self.precision == F32 else torch.float64
+            for t, nm in ((ans, "ans"), (rhs, "rhs")):
+                if t.dtype != want or not t.is_cuda or not t.is_contiguous():
+                    raise TypeError(f"{nm} must be a contiguous CUDA {want} tensor")
             pa, pr = C.c_void_p(ans.data_ptr()), C.c_void_p(rhs.data_ptr())
             st = C.c_void_p(torch.cuda.current_stream(ans.device).cuda_stream)
+        else:
+            raise TypeError("solve_batch takes two fdm_amd.hip.DeviceArray or two torch CUDA tensors")
         _check(_lib_handle().fdmgpu_solve_batch_device(self._h, count, pa, pr, st), f"{self._kind}.solve_batch")
         return ans
 
@@ -293,7 +302,17 @@ class _Plan:
 
     def solve_table(self, ans_tab, rhs_tab, count: int, stream=None):
         """Batch of `count` problems at device pointers ans_tab[p] / rhs_tab[p]
-        (device arrays of uint64 pointers, e.g. hip.DeviceArray)."""
+        (device arrays of uint64 pointers, e.g. hip.DeviceArray).  Every pointer
+        must address a device array of the plan's shape and element type."""
+        if count < 1:
+            raise ValueError("count must be positive")
+        for t, nm in ((ans_tab, "ans_tab"), (rhs_tab, "rhs_tab")):
+            if not _is_dev(t):
+                raise TypeError(f"{nm} must be an fdm_amd.hip.DeviceArray of device pointers")
+            if np.dtype(t.dtype).itemsize != 8 or np.dtype(t.dtype).kind not in "ui":
+                raise TypeError(f"{nm} must hold 64-bit pointers (uint64)")
+            if int(np.prod(t.shape)) < count:
+                raise ValueError(f"{nm} holds {int(np.prod(t.shape))} pointers, fewer than count={count}")
         _check(_lib_handle().fdmgpu_solve_table_device(self._h, count, ans_tab.ptr, rhs_tab.ptr,
                                                        _stream_handle(stream)), f"{self._kind}.solve_table")
 
@@ -439,6 +458,8 @@ class NSCube:
         self.time_index += count
 
     def get(self, name: str) -> np.ndarray:
+        """A copy of field `name`.  Raises FdmGpuError (ERR_NOT_CONVERGED) when
+        a FP32+refine projection of this run ended above its tolerance."""
         out = np.empty(self.shapes[name], np.float64)
         _check(_lib_handle().fdmgpu_ns_get(self._h, NS_FIELDS.index(name), out.ctypes.data_as(C.c_void_p),
                                            out.size), "NSCube.get")
@@ -450,6 +471,13 @@ class NSCube:
             raise ValueError(f"{name} has shape {self.shapes[name]}")
         _check(_lib_handle().fdmgpu_ns_set(self._h, NS_FIELDS.index(name), a.ctypes.data_as(C.c_void_p), a.size),
                "NSCube.set")
+
+    def set_solver_refine(self, rel_tol: float = 1e-10, max_iter: int = 4):
+        """Refinement settings of the FP32+refine pressure solver (fdmgpu_set_refine
+        on the plan of fdmgpu_ns_solver); steps enqueued afterwards use them."""
+        plan = C.c_void_p()
+        _check(_lib_handle().fdmgpu_ns_solver(self._h, C.byref(plan)), "NSCube.solver")
+        _check(_lib_handle().fdmgpu_set_refine(plan, rel_tol, max_iter), "NSCube.set_solver_refine")
 
     def solver_stats(self) -> Stats:
         plan = C.c_void_p()

@@ -607,7 +607,10 @@ int solve_device_impl(fdmgpu_plan_t* P, void* ans, Tab ans_tab, const void* rhs,
                       hipStream_t s) {
     P->last_stream = s;
     P->last_count = count;
-    return plan_solve(P, P->w, ans, ans_tab, rhs, rhs_tab, count, s);
+    const int rc = plan_solve(P, P->w, ans, ans_tab, rhs, rhs_tab, count, s);
+    P->last_iters = P->w.iters;
+    P->last_resid = P->w.resid;
+    return rc;
 }
 
 }  // namespace
@@ -1049,8 +1052,11 @@ int fdmgpu_get_stats(fdmgpu_plan plan, fdmgpu_stats* st) {
         const int c = plan->last_count;
         std::vector<int> it(c);
         std::vector<double> r(c);
-        HIPCHK(hipMemcpy(it.data(), plan->w.iters, sizeof(int) * c, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(r.data(), plan->w.resid, sizeof(double) * c, hipMemcpyDeviceToHost));
+        const int* d_it = plan->last_iters ? plan->last_iters : plan->w.iters;
+        const double* d_r = plan->last_resid ? plan->last_resid : plan->w.resid;
+        if (!d_it || !d_r) return FDMGPU_ERR_INVALID_ARG;
+        HIPCHK(hipMemcpy(it.data(), d_it, sizeof(int) * c, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(r.data(), d_r, sizeof(double) * c, hipMemcpyDeviceToHost));
         st->iterations = *std::max_element(it.begin(), it.end());
         st->rel_residual = r[c - 1];
         st->max_rel_residual = *std::max_element(r.begin(), r.end());

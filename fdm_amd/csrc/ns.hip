@@ -209,6 +209,12 @@ __global__ void ns_update_kernel(Grid g, Fields f) {
     as_global(f.p)[g.p(i, k, j)] = xc;
 }
 
+// Marks the run when the pressure solve's final relative residual (the
+// refinement's per-problem result) is above the tolerance or not a number.
+__global__ void ns_converged_kernel(const double* __restrict__ resid, const double tol, int* __restrict__ bad) {
+    if (threadIdx.x == 0 && !(resid[0] <= tol)) bad[0] = 1;
+}
+
 }  // namespace
 
 struct fdmgpu_ns_t {
@@ -224,6 +230,7 @@ struct fdmgpu_ns_t {
     long steps = 0;
     bool warm_ok = false;          // x holds the pressure of the previous step (a field upload clears it)
     bool use_graph = true;
+    int* d_bad = nullptr;          // set on the device when a projection misses its tolerance (sticky)
 };
 
 namespace {
@@ -260,10 +267,17 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s) {
     // 2 and, as a guard that costs only empty launches once converged, 3)
     // instead of solving from zero.  The tolerance and the stopping test are
     // those of a cold solve.
-    const bool warm = N->plan->precision == FDMGPU_F32_REFINE64 && N->warm_ok;
-    int rc = warm ? plan_solve_iters(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s, 2, 3)
+    const bool refine = N->plan->precision == FDMGPU_F32_REFINE64;
+    const bool warm = refine && N->warm_ok;
+    int rc = warm ? plan_solve_iters(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s, 2,
+                                     std::max(2, N->plan->max_iter - 1))
                   : plan_solve(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s);
     if (rc) return rc;
+    if (refine) {   // a projection that missed the tolerance marks the run (no host round trip)
+        hipLaunchKernelGGL(ns_converged_kernel, dim3(1), dim3(64), 0, s, (const double*)N->plan->w.resid,
+                           N->plan->rtol, N->d_bad);
+        NS_HIPCHK(hipGetLastError());
+    }
     hipLaunchKernelGGL(ns_update_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
     NS_HIPCHK(hipGetLastError());
     return FDMGPU_OK;
@@ -277,6 +291,7 @@ void ns_free(fdmgpu_ns_t* N) {
     if (N->graph) (void)hipGraphDestroy(N->graph);
     double* a[] = {N->f.u, N->f.v, N->f.w, N->f.p, N->f.x, N->f.F, N->f.G, N->f.H, N->f.RHS};
     for (double* p : a) (void)hipFree(p);
+    (void)hipFree(N->d_bad);
     if (N->plan) fdmgpu_destroy(N->plan);
     if (N->ev_in) (void)hipEventDestroy(N->ev_in);
     if (N->ev_out) (void)hipEventDestroy(N->ev_out);
@@ -324,6 +339,8 @@ int fdmgpu_ns_create(fdmgpu_ns* ns, int solver_precision, int nx, int ny, int nz
     // the solver of NSCube::lapl_solver (src/ns_cube.h:77)
     if (!rc) rc = fdmgpu_plan_cube(&N->plan, solver_precision, 0, g.dx, g.dy, g.dz, x2 - x1 + g.dx, y2 - y1 + g.dy,
                                    z2 - z1 + g.dz, nx, ny, nz);
+    if (!rc && (hipMalloc(&N->d_bad, sizeof(int)) != hipSuccess || hipMemset(N->d_bad, 0, sizeof(int)) != hipSuccess))
+        rc = FDMGPU_ERR_OUT_OF_MEMORY;
     if (!rc && (hipStreamCreateWithFlags(&N->cap, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&N->ev_in, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&N->ev_out, hipEventDisableTiming) != hipSuccess))
@@ -372,6 +389,8 @@ int fdmgpu_ns_step(fdmgpu_ns N, int count, void* stream) {
     if (N->steps > 0) {   // fdmgpu_get_stats on the solver plan reports the last step's pressure solve
         N->plan->last_stream = s;
         N->plan->last_count = 1;
+        N->plan->last_iters = N->plan->w.iters;
+        N->plan->last_resid = N->plan->w.resid;
     }
     NS_HIPCHK(hipEventRecord(N->ev_out, s));
     NS_HIPCHK(hipStreamWaitEvent(user, N->ev_out, 0));
@@ -385,7 +404,9 @@ int fdmgpu_ns_get(fdmgpu_ns N, int which, double* host, unsigned long long count
     NS_HIPCHK(hipSetDevice(N->device));
     NS_HIPCHK(hipStreamSynchronize(N->cap));
     NS_HIPCHK(hipMemcpy(host, *p, sizeof(double) * count, hipMemcpyDeviceToHost));
-    return FDMGPU_OK;
+    int bad = 0;
+    NS_HIPCHK(hipMemcpy(&bad, N->d_bad, sizeof(int), hipMemcpyDeviceToHost));
+    return bad ? FDMGPU_ERR_NOT_CONVERGED : FDMGPU_OK;
 }
 
 int fdmgpu_ns_set(fdmgpu_ns N, int which, const double* host, unsigned long long count) {

@@ -82,6 +82,8 @@ struct fdmgpu_plan_t {
     hipStream_t own = nullptr;
     hipStream_t last_stream = nullptr;
     int last_count = 0;
+    const int* last_iters = nullptr;    // per-problem results of the last refine solve (device):
+    const double* last_resid = nullptr; //   in the scratch that solve used
     // per-kernel timing: bit k set = launches of kernel class k are timed
     // (fdmgpu_set_timing, fdmgpu_set_timing_kernels)
     unsigned timing = 0;

@@ -81,10 +81,12 @@ int make_plan(const Key& k, fdmgpu_plan* out) {
     return FDMGPU_ERR_INVALID_ARG;
 }
 
-int check_problem(const fdmgpu_problem& p) {
+// allow_alias: the autotuner, which solves on copies, accepts d_ans == d_rhs
+// for every precision.
+int check_problem(const fdmgpu_problem& p, bool allow_alias = false) {
     if (p.kind != FDMGPU_KIND_CUBE && p.kind != FDMGPU_KIND_RECT) return FDMGPU_ERR_INVALID_ARG;
     if (!p.d_ans || !p.d_rhs) return FDMGPU_ERR_INVALID_ARG;
-    if (p.precision == FDMGPU_F32_REFINE64 && p.d_ans == p.d_rhs) return FDMGPU_ERR_INVALID_ARG;
+    if (!allow_alias && p.precision == FDMGPU_F32_REFINE64 && p.d_ans == p.d_rhs) return FDMGPU_ERR_INVALID_ARG;
     return FDMGPU_OK;
 }
 
@@ -562,6 +564,10 @@ void worker_main(fdmgpu_server_t* S) {
             }
             lk.unlock();
             const int rc = launch_batch(S, b);
+            // a launch that failed part way may have queued copies and kernels
+            // that still use the batch's scratch and pinned tables: they must
+            // finish before either is handed to another batch
+            if (rc) (void)hipStreamSynchronize(S->streams[b->stream]);
             lk.lock();
             if (rc) {
                 const Clock::time_point now = Clock::now();
@@ -782,7 +788,7 @@ int fdmgpu_tune_set(const fdmgpu_problem* problem, const fdmgpu_variant* v) {
 int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char* cache_path) {
     if (count < 1 || !problems || reps < 1 || reps > 1000) return FDMGPU_ERR_INVALID_ARG;
     for (int i = 0; i < count; i++) {
-        int rc = check_problem(problems[i]);
+        int rc = check_problem(problems[i], true);
         if (rc) return rc;
     }
     // problems grouped by shape key; each shape tuned on its own batch
@@ -803,17 +809,28 @@ int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char*
         const int n = (int)ids.size();
         fdmgpu_plan plan = nullptr;
         if ((rc = make_plan(key_of(problems[ids[0]]), &plan))) break;
+        // every variant runs on copies of the shape's right-hand sides and
+        // writes into scratch answers: the caller's arrays are only read
+        // (a problem may even alias ans and rhs)
+        const size_t pb = (size_t)plan_elems(plan) * plan->elem;
+        const size_t pba = (pb + 255) & ~(size_t)255;
         void** d_tab = nullptr;
+        char* d_copy = nullptr;
         std::vector<const void*> tab(2 * n);
-        for (int j = 0; j < n; j++) {
-            tab[j] = problems[ids[j]].d_ans;
-            tab[n + j] = problems[ids[j]].d_rhs;
+        bool ok = hipMalloc(&d_copy, 2 * pba * n) == hipSuccess && hipMalloc(&d_tab, sizeof(void*) * 2 * n) == hipSuccess;
+        for (int j = 0; j < n && ok; j++) {
+            tab[j] = d_copy + pba * j;
+            tab[n + j] = d_copy + pba * (n + j);
+            ok = hipMemcpyAsync(const_cast<void*>(tab[n + j]), problems[ids[j]].d_rhs, pb, hipMemcpyDeviceToDevice,
+                                s) == hipSuccess;
         }
-        if (hipMalloc(&d_tab, sizeof(void*) * 2 * n) != hipSuccess ||
-            hipMemcpy(d_tab, tab.data(), sizeof(void*) * 2 * n, hipMemcpyHostToDevice) != hipSuccess) {
+        if (ok) ok = hipMemcpyAsync(d_tab, tab.data(), sizeof(void*) * 2 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+                     hipStreamSynchronize(s) == hipSuccess;
+        if (!ok) {
             rc = FDMGPU_ERR_OUT_OF_MEMORY;
             fdmgpu_destroy(plan);
             (void)hipFree(d_tab);
+            (void)hipFree(d_copy);
             break;
         }
         double best_ms = 1e300;
@@ -838,6 +855,7 @@ int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char*
         }
         (void)hipStreamSynchronize(s);
         (void)hipFree(d_tab);
+        (void)hipFree(d_copy);
         fdmgpu_destroy(plan);
         if (rc) break;
         found[kv.first] = {best, best_ms};

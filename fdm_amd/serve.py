@@ -107,8 +107,9 @@ class Server:
 
 
 def tune(problems, reps: int = 5, cache_path: str | None = None):
-    """Time every kernel variant of each shape among `problems` (their ans arrays
-    are used as scratch) and keep the fastest (process-wide; JSON file if given)."""
+    """Time every kernel variant of each shape among `problems` (on device copies
+    of their right-hand sides; the problems' own arrays are only read) and keep
+    the fastest (process-wide; JSON file if given)."""
     arr = (Problem * len(problems))(*problems)
     _check(_lib_handle().fdmgpu_tune(len(problems), arr, reps, cache_path.encode() if cache_path else None), "tune")
 

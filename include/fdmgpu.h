@@ -252,7 +252,9 @@ int fdmgpu_server_forget(fdmgpu_server server, unsigned long long ticket);
 int fdmgpu_server_destroy(fdmgpu_server server);
 
 /* Autotuner: times the kernel variants of every solver in `problems` (each
- * solved `reps` times per variant on scratch copies) and records the fastest
+ * solved `reps` times per variant on device copies of its right-hand side,
+ * into scratch answers: the problems' own arrays are only read, and a
+ * problem may alias d_ans and d_rhs) and records the fastest
  * in the process-wide table, and in the JSON file `cache_path` when not NULL
  * (read back by later processes through FDMGPU_TUNE_CACHE). */
 int fdmgpu_tune(int count, const fdmgpu_problem* problems, int reps, const char* cache_path);
@@ -291,7 +293,13 @@ int fdmgpu_ns_step(fdmgpu_ns ns, int count, void* stream);
 #define FDMGPU_NS_G 6
 #define FDMGPU_NS_H 7
 #define FDMGPU_NS_RHS 8
-/* Copy a field to / from host memory (synchronises); count = its element count. */
+/* Copy a field to / from host memory (synchronises); count = its element count.
+ * With the FP32+refine pressure solve, fdmgpu_ns_get copies the field and
+ * then returns FDMGPU_ERR_NOT_CONVERGED if any projection so far ended above
+ * the solver's tolerance (a flag kept on the device by every step, replayed
+ * ones included; it stays set for the rest of the run).  Warm-started
+ * projections run inner solves 2 .. max(2, max_iter - 1) of the solver plan's
+ * refinement settings as they were when the step was first enqueued. */
 int fdmgpu_ns_get(fdmgpu_ns ns, int which, double* host, unsigned long long count);
 int fdmgpu_ns_set(fdmgpu_ns ns, int which, const double* host, unsigned long long count);
 /* Element count and device address of a field (device-side post-processing). */

@@ -87,6 +87,26 @@ def test_ns_refine_field_upload(fdm):
         assert relmax(g.get(f), o.get(f)) <= 1e-8, f
 
 
+def test_ns_refine_not_converged_is_reported(fdm):
+    """A projection that misses its tolerance (1e-30 in two inner solves, steps
+    replayed from the graph included) is reported by the next field copy as
+    ERR_NOT_CONVERGED; the same run with the default settings reports none."""
+    n = 15
+    say("ns refine not converged flag", n)
+    kw = dict(u0=1.0, Re=250.0, dt=0.01)
+    ok = fdm.NSCube(n, n, n, solver="refine", **kw)
+    ok.step(4)
+    assert np.isfinite(ok.get("u")).all()
+    bad = fdm.NSCube(n, n, n, solver="refine", **kw)
+    bad.set_solver_refine(1e-30, 3)
+    bad.step(4)
+    with pytest.raises(fdm.FdmGpuError) as e:
+        bad.get("u")
+    assert e.value.code == fdm.ERR_NOT_CONVERGED
+    ok.close()
+    bad.close()
+
+
 def test_ns_errors(fdm):
     say("ns errors")
     with pytest.raises(fdm.FdmGpuError) as e:      # lid loop bound nz+1 beyond u's x range
