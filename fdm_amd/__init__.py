@@ -27,7 +27,7 @@ _PREC = {"f32": F32, "float": F32, "f64": F64, "double": F64, "refine": F32_REFI
 
 # error codes (include/fdmgpu.h)
 OK, ERR_INVALID_SIZE, ERR_INVALID_ARG, ERR_UNSUPPORTED = 0, 1, 2, 3
-ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_ADMISSION = 4, 5, 6, 7, 8
+ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_ADMISSION, ERR_TOO_LARGE = 4, 5, 6, 7, 8, 9
 
 # every entry point include/fdmgpu.h declares
 EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdmgpu_plan_cyl3_fft2",
@@ -39,7 +39,7 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdm
            "fdmgpu_server_submit", "fdmgpu_server_set_refine", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
            "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_tune_set", "fdmgpu_ns_create", "fdmgpu_ns_step", "fdmgpu_ns_get", "fdmgpu_ns_set", "fdmgpu_ns_info",
            "fdmgpu_ns_solver", "fdmgpu_ns_destroy", "fdmgpu_destroy", "fdmgpu_strerror",
-           "fdmgpu_version")
+           "fdmgpu_version", "fdmgpu_solve_batch_noalloc")
 
 # kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
 K_XY_FWD, K_XY_INV, K_X_TILE, K_Y_TILE, K_ZCR, K_ROWCR, K_RESIDUAL, K_OTHER = range(8)
@@ -111,6 +111,7 @@ def _lib_handle():
         L.fdmgpu_solve_host.argtypes = [vp, vp, vp]
         L.fdmgpu_solve_device.argtypes = [vp, vp, vp, vp]
         L.fdmgpu_solve_batch_device.argtypes = [vp, i, vp, vp, vp]
+        L.fdmgpu_solve_batch_noalloc.argtypes = [vp, i, vp, vp, vp]
         L.fdmgpu_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.fdmgpu_plan_footprint.argtypes = [vp, i, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
         L.fdmgpu_set_timing.argtypes = [vp, i]
@@ -238,6 +239,15 @@ class _Plan:
     def solve_batch(self, ans, rhs, stream=None):
         """count problems stacked on a leading axis (DeviceArray or torch CUDA tensors),
         solved in one launch sequence."""
+        return self._batch(ans, rhs, stream, "fdmgpu_solve_batch_device", "solve_batch")
+
+    def solve_batch_noalloc(self, ans, rhs, stream=None):
+        """solve_batch with no device allocation (fdmgpu_solve_batch_noalloc): the
+        refinement scratch comes from the library's static 64 MiB array, shared by
+        every call on the GPU (calls must not overlap); ERR_TOO_LARGE beyond it."""
+        return self._batch(ans, rhs, stream, "fdmgpu_solve_batch_noalloc", "solve_batch_noalloc")
+
+    def _batch(self, ans, rhs, stream, entry, what):
         count = ans.shape[0]
         if tuple(ans.shape[1:]) != self.shape or tuple(rhs.shape) != tuple(ans.shape):
             raise ValueError("batch shape mismatch")
@@ -255,8 +265,8 @@ class _Plan:
             pa, pr = C.c_void_p(ans.data_ptr()), C.c_void_p(rhs.data_ptr())
             st = C.c_void_p(torch.cuda.current_stream(ans.device).cuda_stream)
         else:
-            raise TypeError("solve_batch takes two fdm_amd.hip.DeviceArray or two torch CUDA tensors")
-        _check(_lib_handle().fdmgpu_solve_batch_device(self._h, count, pa, pr, st), f"{self._kind}.solve_batch")
+            raise TypeError(f"{what} takes two fdm_amd.hip.DeviceArray or two torch CUDA tensors")
+        _check(getattr(_lib_handle(), entry)(self._h, count, pa, pr, st), f"{self._kind}.{what}")
         return ans
 
     def set_timing(self, enable: bool = True, kernels=None):

@@ -762,6 +762,7 @@ const char* fdmgpu_strerror(int code) {
         case FDMGPU_ERR_SINGULAR: return g_errbuf[0] ? g_errbuf : "tridiagonal system needs pivoting (gtsv info != 0)";
         case FDMGPU_ERR_NOT_CONVERGED: return "refinement did not reach the requested residual";
         case FDMGPU_ERR_ADMISSION: return "request exceeds the HBM budget";
+        case FDMGPU_ERR_TOO_LARGE: return g_errbuf[0] ? g_errbuf : "solve needs more than the static scratch";
         default: return "unknown error";
     }
 }

@@ -38,6 +38,7 @@ typedef struct fdmgpu_plan_t* fdmgpu_plan;
 #define FDMGPU_ERR_SINGULAR 6      /* tridiagonal system not solvable without pivoting: gtsv info != 0, src/lapl_rect.cpp:91 */
 #define FDMGPU_ERR_NOT_CONVERGED 7 /* FP32+refine did not reach the requested residual */
 #define FDMGPU_ERR_ADMISSION 8     /* serving: request larger than the HBM budget */
+#define FDMGPU_ERR_TOO_LARGE 9     /* fdmgpu_solve_batch_noalloc: scratch beyond the static scratch array */
 
 /* ---- precision -----------------------------------------------------------*/
 #define FDMGPU_F32 0          /* float in/out, float arithmetic: LaplCube<float>/LaplRect<float> */
@@ -123,6 +124,20 @@ int fdmgpu_solve_batch_device(fdmgpu_plan plan, int count, void* d_ans, const vo
  * pointer.  No reference counterpart. */
 int fdmgpu_solve_table_device(fdmgpu_plan plan, int count, void* const* d_ans_tab, const void* const* d_rhs_tab,
                               void* stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same solve as fdmgpu_solve_batch_device (same
+ * kernels, bitwise the same results) with no device allocation.  The
+ * FP32+refine scratch (and, for an in-place refine solve, the copy of the
+ * right-hand side) comes from a static device array of 64 MiB that each GPU
+ * the library runs on holds until the process ends; a solve that needs more
+ * returns FDMGPU_ERR_TOO_LARGE before enqueueing anything (a 127^3 refine
+ * problem needs about 16.5 MB, so three fit; plain F32/F64 solves need no
+ * scratch).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU race (order them, e.g. on one stream).  fdmgpu_get_stats reports
+ * the call until another call on the GPU overwrites the scratch. */
+int fdmgpu_solve_batch_noalloc(fdmgpu_plan plan, int count, void* d_ans, const void* d_rhs, void* stream);
 
 /* Kernel variants (tile shapes) of a plan, as the autotuner chooses them:
  * plane = 1 runs square planes that fit LDS through the plane kernel (0: x
