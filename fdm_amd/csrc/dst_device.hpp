@@ -165,6 +165,32 @@ struct DstLane {
             sp[k2] = tables[2 * N + t + N1 * k2];
         }
     }
+    __device__ __forceinline__ T sine(const int j2) const { return sp[j2]; }
+};
+
+// The same constants with the shifted sines read from a per-block LDS copy of
+// the table at each use instead of held in registers (N2 registers of T fewer
+// per lane): for kernels whose register budget is taken by other work.
+template <typename T, int LOGN>
+struct DstLaneLdsSine {
+    using S = DstShape<LOGN>;
+    T twr[S::N2], twi[S::N2];
+    const T* sl;   // LDS: sin(pi m/N) + 1/2, m < N, from the lane's own m = t
+    // lds_sines: N values of the block's LDS, filled by fill_sines before the
+    // first barrier
+    __device__ __forceinline__ void init(const T* __restrict__ tables, const T* lds_sines, const int t) {
+        constexpr int N = S::N, N2 = S::N2;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            twr[k2] = tables[t * k2];
+            twi[k2] = tables[N + t * k2];
+        }
+        sl = lds_sines + t;
+    }
+    __device__ static void fill_sines(const T* __restrict__ tables, T* lds_sines, const int tid, const int nt) {
+        for (int m = tid; m < S::N; m += nt) lds_sines[m] = tables[2 * S::N + m];
+    }
+    __device__ __forceinline__ T sine(const int j2) const { return sl[S::N1 * j2]; }
 };
 
 // LDS layouts of a set of pencils.  Measured on gfx950: b32 accesses and b64
@@ -275,8 +301,8 @@ struct PencilIO {
 // kRdftInv: C_k = X[k]/2 - i X[N-k]/2 (Hermitian; C_0, C_{N/2} real),
 //   w = C^a + i C^b, x^a + i x^b = sum_k w_k e^{+2 pi i jk/N}
 //   = conj(FFT(conj w)); step 2 writes the result directly.
-template <int KIND, typename T, int LOGN, int JOBS, typename Lay, int NT>
-__device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<T, LOGN>& L, const T scale,
+template <int KIND, typename T, int LOGN, int JOBS, typename Lay, int NT, typename Lane = DstLane<T, LOGN>>
+__device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const Lane& L, const T scale,
                                              const int tid) {
     using S = DstShape<LOGN>;
     constexpr int N = S::N, N1 = S::N1, N2 = S::N2, KC = S::KC;
@@ -308,7 +334,7 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const DstLane<
                     const T a0 = PA(t, N1 * j2), a1 = PA(jl, jc);
                     const T b0 = PB(t, N1 * j2), b1 = PB(jl, jc);
                     if constexpr (KIND == kDst) {
-                        const T s = L.sp[j2];
+                        const T s = L.sine(j2);
                         yr[j2] = fma(s, a0 + a1, -a1);
                         yi[j2] = fma(s, b0 + b1, -b1);
                     } else if (2 * j2 < N2) {   // 0 <= j < N/2
@@ -468,8 +494,8 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, 
 // H[c] + (8 slots) * (constant part of m), H[c] = offset of slot t ^ c: one
 // add per distinct (base, c) where the generic path computes the slot and
 // its swizzle per access.
-template <typename T, int P, int JOBS, int NT, bool COLS>
-__device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const DstLane<T, 7>& L, const T scale,
+template <typename T, int P, int JOBS, int NT, bool COLS, typename Lane = DstLane<T, 7>>
+__device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const Lane& L, const T scale,
                                              const int tid) {
     using S = DstShape<7>;
     constexpr int N = 128, N1 = 8, N2 = 16, KC = 8;
@@ -505,7 +531,7 @@ __device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const DstLane<
         const int o0 = b0 + H[0] + ROW8 * j2;
         const int o1 = j2 == 0 ? rA : rB + ROW8 * (15 - j2);
         const T a0 = A(o0), a1 = A(o1), c0 = B(o0), c1 = B(o1);
-        const T sp = L.sp[j2];
+        const T sp = L.sine(j2);
         yr[j2] = fma(sp, a0 + a1, -a1);
         yi[j2] = fma(sp, c0 + c1, -c1);
     }

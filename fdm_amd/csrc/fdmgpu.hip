@@ -204,6 +204,14 @@ bool plane_capable(const fdmgpu_plan_t* P) {
     return P->precision == FDMGPU_F64 ? plane_fits<double>(P->logx) : plane_fits<float>(P->logx);
 }
 
+// Plane kernel of the default variant: the streaming kernel for plain FP64
+// planes of N = 128 (one plane fills a CU's LDS), else the plane kernel where
+// the plane fits LDS, else the axis tiles.
+int default_plane(const fdmgpu_plan_t* P) {
+    if (!plane_capable(P)) return 0;
+    return plane_stream_ok(P->logx, P->precision == FDMGPU_F64) ? 2 : 1;
+}
+
 std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
     const bool f64 = P->precision == FDMGPU_F64;
     auto tiles = [&](int logn) {
@@ -219,10 +227,16 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
             if (zcr_gmul_ok(P->logz, gm)) gms.push_back(gm);
     std::vector<Variant> out;
     Variant d;
-    d.plane = plane_capable(P) ? 1 : 0;
+    d.plane = default_plane(P);
     out.push_back(d);   // the default first
     const bool refine = P->precision == FDMGPU_F32_REFINE64;
     for (int gm : gms) {
+        if (plane_capable(P) && plane_stream_ok(P->logx, P->precision == FDMGPU_F64)) {
+            Variant v;   // plane_stream_kernel (plain FP64 planes of N = 128)
+            v.plane = 2;
+            v.zcr_gmul = gm;
+            out.push_back(v);
+        }
         if (plane_capable(P)) {
             // fuse selects the refinement's fused residual pass: plain plans
             // list 0 only, the value of their default
@@ -330,6 +344,7 @@ int stage_fwd(fdmgpu_plan_t* P, const void* in, Tab in_tab, void* out, Tab out_t
     }
     if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
         PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done, P->kfwd, in_tab, out_tab};
+        a.stream_planes = P->var.plane == 2;
         Timed tm(P, plane_class, s);
         HIPCHK((launch_plane<Tin, T, T>(P->logx, a)));
         return FDMGPU_OK;
@@ -387,6 +402,7 @@ int stage_inv(fdmgpu_plan_t* P, void* in, Tab in_tab, void* out, Tab out_tab, in
     if (P->kind == 3 && P->var.plane && P->logx == P->logy && plane_fits<T>(P->logx)) {
         PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv, in_tab, out_tab,
                     base};
+        a.stream_planes = P->var.plane == 2;
         Timed tm(P, accum ? FDMGPU_K_XY_INV_ACC : FDMGPU_K_XY_INV, s);
         HIPCHK((launch_plane<T, Tout, T>(P->logx, a)));
         return FDMGPU_OK;
@@ -812,7 +828,7 @@ int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double
         free_plan(P);
         return rc;
     }
-    P->var.plane = plane_capable(P) ? 1 : 0;   // the default variant
+    P->var.plane = default_plane(P);   // the default variant
     apply_tuned(P);
     *plan = P;
     return FDMGPU_OK;
@@ -853,7 +869,7 @@ int fdmgpu_plan_rect(fdmgpu_plan* plan, int precision, int periodic_mask, double
         free_plan(P);
         return rc;
     }
-    P->var.plane = plane_capable(P) ? 1 : 0;   // the default variant
+    P->var.plane = default_plane(P);   // the default variant
     apply_tuned(P);
     *plan = P;
     return FDMGPU_OK;

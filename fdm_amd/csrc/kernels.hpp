@@ -29,6 +29,18 @@ struct BatchPtr {
     }
 };
 
+// A pointer every lane of the wave holds (e.g. read from a per-problem table
+// at a wave-uniform index), moved to scalar registers: addresses derived from
+// it need no per-lane arithmetic, and a buffer resource built on it needs no
+// loop over distinct values.
+template <typename T>
+__device__ __forceinline__ T* wave_uniform(T* p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
 // The same address as a global-memory pointer: loads and stores through it are
 // global_* instructions, which cannot alias LDS, so the compiler can keep many
 // in flight (through a generic pointer they are flat_* and every one is
@@ -88,6 +100,7 @@ struct PlaneArgs {
     const void* const* out_tab;
     const float* base = nullptr;  // with accum: out = base + result (base contiguous, stride bs_out);
                                   // problems flagged done get out = base
+    bool stream_planes = false;   // plane_stream_kernel: FP64, N = 128, plain (no accum / base / done)
 };
 
 struct AxisArgs {
@@ -183,6 +196,8 @@ template <typename Tu>
 hipError_t launch_plane_resid(int logn, const PlaneResidArgs& a);
 template <typename T>
 bool plane_fits(int logn);
+// plane_stream_kernel exists for this plane (FP64 planes of N = 128)
+bool plane_stream_ok(int logn, bool f64);
 template <typename Tin, typename Tout, typename T>
 hipError_t launch_xaxis(int logn, const AxisArgs& a);
 template <typename Tin, typename Tout, typename T>

@@ -31,7 +31,8 @@ using Tab = const void* const*;
 
 // Kernel variant of a plan (chosen by the autotuner; 0 = built-in default).
 struct Variant {
-    int plane = 1;      // square planes that fit LDS: 1 = plane kernel, 0 = x and y tile kernels
+    int plane = 1;      // square planes that fit LDS: 1 = plane kernel, 0 = x and y tile kernels,
+                        // 2 = streaming plane kernel (plain FP64 planes of N = 128)
     int tile_x = 0;     // x tile kernel: rows per block (16/32/64)
     int tile_y = 0;     // y tile kernel: columns per block
     int zcr_gmul = 0;   // z cyclic reduction: lanes-per-system multiplier (1/2)
@@ -120,6 +121,8 @@ inline long plan_elems(const fdmgpu_plan_t* P) { return (long)P->nx * P->ny * P-
 void set_error_text(const char* msg);
 // Whether plan P can use the plane kernel (square planes that fit LDS).
 bool plane_capable(const fdmgpu_plan_t* P);
+// plane field of P's default variant (0 axis tiles, 1 plane kernel, 2 streaming plane kernel)
+int default_plane(const fdmgpu_plan_t* P);
 // Kernel variants valid for P (the first is the default).
 std::vector<Variant> plan_variants(const fdmgpu_plan_t* P);
 bool variant_valid(const fdmgpu_plan_t* P, const Variant& v);

@@ -175,6 +175,110 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
 }
 
 // ---------------------------------------------------------------------------
+// Streaming plane kernel for FP64 planes of N = 128 (the plain LaplCube<double>
+// solve at 127^3, periodic 128^3).  Such a plane takes 139 KB of LDS, so one
+// block fills a CU; the one-plane-per-block kernel then leaves HBM idle while
+// the block transforms and the CU idle while it loads.  Here a block walks
+// planes q = blockIdx.x, blockIdx.x + gridDim.x, ... (about the same count per
+// block: the grid is sized so), and right after plane q is staged into LDS it
+// issues the global loads of its next plane into registers; they are in
+// flight during the x and y passes and the store phase of plane q.  The
+// transforms, the LDS layout and the order of every floating-point operation
+// are those of plane_dst_kernel, so results are bitwise the same.
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu(2))) void plane_stream_kernel(
+    const BatchPtr<const double> in, const BatchPtr<double> out, const double sx, const double sy,
+    const double* __restrict__ tables, const int nplanes, const int total) {
+    constexpr int LOGN = 7;
+    using C = PlaneCfg<LOGN>;
+    using S = DstShape<LOGN>;
+    using IO = PencilIO<KIND, LOGN>;
+    using LX = PlaneLayout<C::P, S::N1, true>;
+    using LY = PlaneLayout<C::P, S::N1, false>;
+    constexpr int N = C::N, n = IO::n, o = IO::off, NT = C::NT;
+    constexpr int R = NT / N, ITER = (n + R - 1) / R;
+    static_assert(NT % N == 0, "a block covers whole rows");
+    const int tid = threadIdx.x;
+    int q = blockIdx.x;
+    if (q >= total) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* lds = reinterpret_cast<double*>(smem_raw);
+    constexpr unsigned plane_pts = (unsigned)n * n;
+    const SweepSlots<LOGN, NT, C::P, S::N1, o> slot(tid);
+    const int x0 = tid & (N - 1), r0 = tid >> LOGN;
+    const int lane_in = (int)((r0 * n + x0) * sizeof(double));
+
+    // the DST's shifted sines come from an LDS copy behind the plane (the
+    // prefetch takes their registers)
+    using Lane = DstLaneLdsSine<double, LOGN>;
+    double* sines = lds + N * C::P;
+    Lane L;
+    Lane::fill_sines(tables, sines, tid, NT);
+    L.init(tables, sines, tid & (S::N1 - 1));
+    if constexpr (KIND == kDst) {   // the dummy row and column stay 0 across planes
+        for (int i = tid; i < N; i += NT) {
+            lds[LX::at(0, i)] = 0.0;
+            lds[LX::at(i, 0)] = 0.0;
+        }
+    }
+    double v[ITER];
+    // (problem, plane) of a plane index, wave-uniform: in SGPRs, so the
+    // buffer resource and the store base need no per-lane handling
+    auto split = [&](const int qq, int& p, int& z) {
+        p = __builtin_amdgcn_readfirstlane(qq / nplanes);
+        z = __builtin_amdgcn_readfirstlane(qq - p * nplanes);
+    };
+    auto load_plane = [&](const int qq) {
+        int p, z;
+        split(qq, p, z);
+        const BufRange<double> src(wave_uniform(in(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
+        if (x0 < n) {
+#pragma unroll
+            for (int i = 0; i < ITER; ++i)
+                if (R * i + R <= n || r0 + R * i < n) v[i] = src.load(lane_in, (int)(R * i * n * sizeof(double)));
+        }
+    };
+    load_plane(q);
+    for (;;) {
+        if (x0 < n) {
+#pragma unroll
+            for (int i = 0; i < ITER; ++i)
+                if (R * i + R <= n || r0 + R * i < n) lds[slot(i)] = v[i];
+        }
+        __syncthreads();
+        const int qn = q + (int)gridDim.x;
+        if (qn < total) load_plane(qn);   // in flight until the next staging
+        __builtin_amdgcn_sched_barrier(0);
+        // the passes' addresses are recomputed per plane from a thread index
+        // the compiler cannot hoist out of the loop: kept live across the
+        // loop they would take the registers the prefetch needs
+        int t1 = tid;
+        asm volatile("" : "+v"(t1));
+        pencil_pairs<KIND, double, LOGN, N / 2, LX, NT, Lane>(lds, L, sx, t1);   // pencils = rows    (x)
+        __syncthreads();
+        int t2 = tid;
+        asm volatile("" : "+v"(t2));
+        if constexpr (KIND == kDst)
+            dst_pairs128<double, C::P, N / 2, NT, true, Lane>(lds, L, sy, t2);   // pencils = columns (y)
+        else
+            pencil_pairs<KIND, double, LOGN, N / 2, LY, NT, Lane>(lds, L, sy, t2);
+        __syncthreads();
+        {
+            int t3 = tid;
+            asm volatile("" : "+v"(t3));
+            int p, z;
+            split(q, p, z);
+            const auto dst = as_global(wave_uniform(out(p)) + (long)z * plane_pts);
+            plane_sweep<LOGN, NT, n>(t3, [&](int, int y, int x) { dst[y * n + x] = lds[LX::at(IO::out(y), IO::out(x))]; });
+        }
+        if (qn >= total) break;
+        q = qn;
+        __syncthreads();   // the store phase has read the plane before the next one is staged
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Refinement: forward DST of the FP64 residual r = f - A u, fused into the
 // plane load.  A is the 7-point operator the solve inverts (1/h^2 per axis:
 // ix, iy, iz).  Each block also writes (sum r^2, sum f^2) of its plane to
@@ -291,9 +395,51 @@ static hipError_t plane_k(const PlaneArgs& a) {
     return hipGetLastError();
 }
 
+// Compute units of the current device (cached per device).
+static int device_cus() {
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c < 1) c = 256;
+        cus[dev] = c;
+    }
+    return cus[dev];
+}
+
+template <int KIND>
+static hipError_t plane_stream_k(const PlaneArgs& a) {
+    using C = PlaneCfg<7>;
+    auto k = plane_stream_kernel<KIND>;
+    const size_t lds = C::template lds_bytes<double>() + sizeof(double) * C::N;   // plane + sine table
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    const long total = (long)a.nplanes * a.batch;
+    if (total < 1 || total > 0x7fffffffL) return hipErrorInvalidValue;
+    // one block per CU; every block gets ceil(total / CUs) planes or one fewer
+    const long per = (total + device_cus() - 1) / device_cus();
+    const int grid = (int)((total + per - 1) / per);
+    const BatchPtr<const double> in{(const double*)a.in, a.in_tab, a.bs_in};
+    const BatchPtr<double> out{(double*)a.out, a.out_tab, a.bs_out};
+    hipLaunchKernelGGL(k, dim3(grid), dim3(C::NT), lds, a.stream, in, out, a.sx, a.sy, (const double*)a.tables,
+                       a.nplanes, (int)total);
+    return hipGetLastError();
+}
+
 template <typename Tin, typename Tout, typename T, int LOGN>
 static hipError_t plane_t(const PlaneArgs& a) {
     if (a.base && !a.accum) return hipErrorInvalidValue;
+    if (a.stream_planes) {   // plain FP64 planes of N = 128 only
+        if constexpr (LOGN == 7 && std::is_same_v<Tin, double> && std::is_same_v<Tout, double> &&
+                      std::is_same_v<T, double>) {
+            if (a.accum || a.base || a.done) return hipErrorInvalidValue;
+            if (a.kind == kDst) return plane_stream_k<kDst>(a);
+            if (a.kind == kRdftFwd) return plane_stream_k<kRdftFwd>(a);
+            if (a.kind == kRdftInv) return plane_stream_k<kRdftInv>(a);
+        }
+        return hipErrorInvalidValue;
+    }
     if (a.kind == kDst) {
         if constexpr (std::is_same_v<Tout, double> && !std::is_same_v<Tin, double>) {
             if (a.base) return plane_k<Tin, Tout, T, LOGN, 2, kDst>(a);
@@ -359,6 +505,8 @@ bool plane_fits(int logn) {
     size_t N = (size_t)1 << logn, N1 = (size_t)1 << (logn / 2);
     return sizeof(T) * (N * (N + N1)) <= 160 * 1024;
 }
+
+bool plane_stream_ok(int logn, bool f64) { return f64 && logn == 7; }
 
 template hipError_t launch_plane<float, float, float>(int, const PlaneArgs&);
 template hipError_t launch_plane<double, double, double>(int, const PlaneArgs&);

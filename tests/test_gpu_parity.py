@@ -338,3 +338,46 @@ def test_errors(fdm):
     with pytest.raises(fdm.FdmGpuError) as e:                 # would need pivoting
         s.set_scales(np.zeros(11), np.full(11, 5.0), np.full(11, 5.0))
     assert e.value.code == fdm.ERR_SINGULAR
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_f64_streaming_planes_vs_plane_kernel(fdm, periodic):
+    """The streaming FP64 plane kernel (default for planes of N = 128) against the
+    one-plane-per-block kernel and the oracle: a batch of 3 (blocks walk two
+    planes each), contiguous and through a pointer table; both within 1e-12 of
+    the oracle."""
+    from fdm_amd import hip
+    n = 128 if periodic else 127
+    say("f64 streaming planes", n, "periodic" if periodic else "dirichlet")
+    h = 2 * np.pi / n
+    l = n * h if periodic else (n + 1) * h
+    F = np.stack([O.field(90 + k, (n, n, n)) for k in range(3)])
+    if periodic:
+        F -= F.mean(axis=(1, 2, 3), keepdims=True)
+    s = fdm.LaplCube(h, h, h, l, l, l, n, n, n, precision="f64", periodic=periodic)
+    assert s.variant[0] == 2   # the default for FP64 planes of N = 128
+    d_f = hip.DeviceArray.from_host(F)
+    out = {}
+    for v in ((2, 0, 0, 0, 0), next(x for x in s.variants() if x[0] == 1)):
+        if v[0] == 2:
+            s.variant = s.variants()[0]
+        else:
+            s.variant = v
+        d_u = hip.DeviceArray(F.shape, np.float64)
+        s.solve_batch(d_u, d_f)
+        hip.synchronize()
+        out[v[0]] = d_u.download()
+    # the same problems through a pointer table, reversed
+    fs = [hip.DeviceArray.from_host(F[i]) for i in range(3)]
+    us = [hip.DeviceArray(F[i].shape, np.float64) for i in range(3)]
+    s.variant = s.variants()[0]
+    tab_u = hip.DeviceArray.from_host(np.array([us[i].ptr.value for i in (2, 1, 0)], np.uint64))
+    tab_f = hip.DeviceArray.from_host(np.array([fs[i].ptr.value for i in (2, 1, 0)], np.uint64))
+    s.solve_table(tab_u, tab_f, 3)
+    hip.synchronize()
+    for i in range(3):
+        assert np.array_equal(us[i].download(), out[2][i]), i
+    for i in (0, 2):
+        ref = O.lapl_cube(F[i], h, h, h, l, l, l, periodic=periodic)
+        assert rel2(out[2][i], ref) <= 1e-12
+        assert rel2(out[1][i], ref) <= 1e-12
