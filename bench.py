@@ -483,7 +483,7 @@ def leg_c3(steps: int = 5, count: int = 512, tune_reps: int = 3):
     return out
 
 
-def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 32, seed: int = 4,
+def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 64, seed: int = 4,
            fp64_sample: int = 120, fracs=(0.25, 0.5, 0.75), rates=None, sat_requests: int = 1200):
     """Serving: Poisson arrivals of the uneven mix + 5% 255^3 (FP32+refine).
     Saturation is the throughput of a burst of `sat_requests` requests (one

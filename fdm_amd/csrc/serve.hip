@@ -437,6 +437,9 @@ struct fdmgpu_server_t {
     int max_batch = 64;
     std::vector<hipStream_t> streams;
     std::vector<int> stream_load;
+    int max_inflight = 2;         // batches in flight per stream
+    unsigned long long waiting_head = 0;   // ticket of the queue's head when it last had to wait
+    int bypassed = 0;             // batches launched before that head since
     char* d_arena = nullptr;
     Arena arena;
     std::mutex mu;
@@ -507,6 +510,27 @@ int launch_batch(fdmgpu_server_t* S, Batch* b) {
     return FDMGPU_OK;
 }
 
+// Problems of at least this many points are "large" (a 255^3 problem has
+// 16.6M; 127^3 has 2.0M): they get stream 0 to themselves.
+constexpr long kLargeElems = 1L << 22;
+// Batches that may launch before a waiting head of the queue.
+constexpr int kMaxBypass = 8;
+
+// Least-loaded stream of the size class of a problem of `elems` points that
+// still takes a batch (stream_load < max_inflight), or -1.
+int pick_stream(const fdmgpu_server_t* S, long elems) {
+    const int ns = (int)S->streams.size();
+    int lo = 0, hi = ns;                   // one stream: every class shares it
+    if (ns > 1) {
+        if (elems >= kLargeElems) hi = 1;  // stream 0
+        else lo = 1;                       // streams 1 .. ns-1
+    }
+    int best = -1;
+    for (int i = lo; i < hi; i++)
+        if (S->stream_load[i] < S->max_inflight && (best < 0 || S->stream_load[i] < S->stream_load[best])) best = i;
+    return best;
+}
+
 void worker_main(fdmgpu_server_t* S) {
     (void)hipSetDevice(S->device);
     std::unique_lock<std::mutex> lk(S->mu);
@@ -514,79 +538,115 @@ void worker_main(fdmgpu_server_t* S) {
         if (S->stop && S->queue.empty() && S->running.empty()) break;
         bool progressed = false;
 
-        // 1. admission + batching: requests in arrival order; a batch takes the
-        //    queued requests of the head's solver (up to max_batch) whose
-        //    scratch fits the arena; the head waits when even it does not fit.
-        while (!S->queue.empty()) {
+        // 1. admission + batching.  Requests are taken in arrival order; a
+        //    batch takes the queued requests of one solver (up to max_batch)
+        //    whose scratch fits the arena.  Back-pressure: a stream takes at
+        //    most max_inflight batches, so under load arrivals wait on the host,
+        //    where the next batch of their solver takes all of them (fewer,
+        //    larger launches instead of a stream queue of tiny batches).  Size
+        //    classes: problems of at least kLargeElems points run on stream 0,
+        //    the others on the remaining streams (with one stream, all share
+        //    it), so small requests never queue behind a large solve on the
+        //    GPU.  Bypass: when the head must wait (no stream of its class
+        //    free, or its scratch does not fit), later requests of other
+        //    solvers may launch before it; while it waits for scratch at most
+        //    kMaxBypass batches do, then admission waits for the head, which
+        //    takes the next scratch that frees up (it cannot starve).
+        for (;;) {
+            if (S->queue.empty()) break;
             Req* head = S->queue.front();
-            const Key k = head->key;
-            const int its = head->its_done;
-            std::vector<Req*> take;
-            for (Req* r : S->queue) {
-                if ((int)take.size() >= S->max_batch) break;
-                if (!(r->key < k) && !(k < r->key) && r->its_done == its) take.push_back(r);
+            if (head->ticket != S->waiting_head) {
+                S->waiting_head = head->ticket;
+                S->bypassed = 0;
             }
-            char* mem = nullptr;
-            size_t bytes = 0;
-            while (!take.empty()) {
-                bytes = batch_bytes(k, (int)take.size());
-                mem = S->arena.alloc(bytes);
-                if (mem) break;
-                take.resize(take.size() / 2);
-            }
-            if (!mem) break;   // head-of-line: wait for completions to free scratch
-            Batch* b = get_batch(S);
-            if (!b) {
-                S->arena.release(mem, bytes);
-                S->worker_error = FDMGPU_ERR_OUT_OF_MEMORY;
-                break;
-            }
-            b->reqs = take;
-            b->key = k;
-            b->plan = S->plans[k];
-            {
-                const int maxit = b->plan->max_iter;
-                const int typ = S->typical.count(k) ? S->typical[k] : maxit;
-                b->first = its + 1;
-                b->last = std::min(maxit, std::max(b->first, typ));
-            }
-            b->mem = mem;
-            b->bytes = bytes;
-            int best = 0;
-            for (size_t i = 1; i < S->streams.size(); i++)
-                if (S->stream_load[i] < S->stream_load[best]) best = (int)i;
-            b->stream = best;
-            S->stream_load[best]++;
-            for (Req* r : take) {
-                S->queue.erase(std::find(S->queue.begin(), S->queue.end(), r));
-                if (r->its_done == 0) r->t_launch = Clock::now();
-                r->res.state = 1;
-            }
-            lk.unlock();
-            const int rc = launch_batch(S, b);
-            // a launch that failed part way may have queued copies and kernels
-            // that still use the batch's scratch and pinned tables: they must
-            // finish before either is handed to another batch
-            if (rc) (void)hipStreamSynchronize(S->streams[b->stream]);
-            lk.lock();
-            if (rc) {
-                const Clock::time_point now = Clock::now();
-                for (Req* r : b->reqs) {
-                    r->res.state = 2;
-                    r->res.status = rc;
-                    r->t_done = now;
-                    S->outstanding--;
+            bool launched = false;
+            bool head_needs_scratch = false;   // the head waits for scratch (not for a stream)
+            std::vector<const Req*> tried;     // first request of each solver already tried this pass
+            for (size_t qi = 0; qi < S->queue.size() && !launched; qi++) {
+                Req* cand = S->queue[qi];
+                // requests behind a head that waits for scratch compete with it
+                // for the arena: at most kMaxBypass batches pass it.  A head
+                // that waits for a stream of its size class is not delayed by
+                // requests of the other class.
+                if (qi > 0 && head_needs_scratch && S->bypassed >= kMaxBypass) break;
+                bool seen = false;
+                for (const Req* t : tried)
+                    if (!(t->key < cand->key) && !(cand->key < t->key) && t->its_done == cand->its_done) seen = true;
+                if (seen) continue;
+                tried.push_back(cand);
+                const Key k = cand->key;
+                const int its = cand->its_done;
+                const int best = pick_stream(S, problem_elems(k));
+                if (best < 0) continue;   // every stream of its class is busy
+                std::vector<Req*> take;
+                for (Req* r : S->queue) {
+                    if ((int)take.size() >= S->max_batch) break;
+                    if (!(r->key < k) && !(k < r->key) && r->its_done == its) take.push_back(r);
                 }
-                S->arena.release(b->mem, b->bytes);
-                S->stream_load[b->stream]--;
-                S->spare.push_back(b);
-                S->cv_done.notify_all();
-            } else {
-                S->running.push_back(b);
+                char* mem = nullptr;
+                size_t bytes = 0;
+                while (!take.empty()) {
+                    bytes = batch_bytes(k, (int)take.size());
+                    mem = S->arena.alloc(bytes);
+                    if (mem) break;
+                    take.resize(take.size() / 2);
+                }
+                if (!mem) {   // its scratch does not fit yet
+                    if (qi == 0) head_needs_scratch = true;
+                    continue;
+                }
+                if (qi > 0 && head_needs_scratch) S->bypassed++;
+                Batch* b = get_batch(S);
+                if (!b) {
+                    S->arena.release(mem, bytes);
+                    S->worker_error = FDMGPU_ERR_OUT_OF_MEMORY;
+                    break;
+                }
+                b->reqs = take;
+                b->key = k;
+                b->plan = S->plans[k];
+                {
+                    const int maxit = b->plan->max_iter;
+                    const int typ = S->typical.count(k) ? S->typical[k] : maxit;
+                    b->first = its + 1;
+                    b->last = std::min(maxit, std::max(b->first, typ));
+                }
+                b->mem = mem;
+                b->bytes = bytes;
+                b->stream = best;
+                S->stream_load[best]++;
+                for (Req* r : take) {
+                    S->queue.erase(std::find(S->queue.begin(), S->queue.end(), r));
+                    if (r->its_done == 0) r->t_launch = Clock::now();
+                    r->res.state = 1;
+                }
+                lk.unlock();
+                const int rc = launch_batch(S, b);
+                // a launch that failed part way may have queued copies and kernels
+                // that still use the batch's scratch and pinned tables: they must
+                // finish before either is handed to another batch
+                if (rc) (void)hipStreamSynchronize(S->streams[b->stream]);
+                lk.lock();
+                if (rc) {
+                    const Clock::time_point now = Clock::now();
+                    for (Req* r : b->reqs) {
+                        r->res.state = 2;
+                        r->res.status = rc;
+                        r->t_done = now;
+                        S->outstanding--;
+                    }
+                    S->arena.release(b->mem, b->bytes);
+                    S->stream_load[b->stream]--;
+                    S->spare.push_back(b);
+                    S->cv_done.notify_all();
+                } else {
+                    S->running.push_back(b);
+                }
+                progressed = true;
+                launched = true;
             }
-            progressed = true;
+            if (!launched) break;   // nothing can launch until completions free streams or scratch
         }
-
         // 2. completions
         for (size_t i = 0; i < S->running.size();) {
             Batch* b = S->running[i];

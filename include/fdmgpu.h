@@ -251,9 +251,16 @@ int fdmgpu_group_destroy(fdmgpu_group group);
  * the HBM budget (an arena of `hbm_budget` bytes allocated once), batched per
  * solver as they are admitted (continuous batching: requests arriving while
  * others run form new batches on other streams) and completed
- * asynchronously.  Solver tables stay resident between requests.  A request
- * whose scratch alone exceeds the budget fails with FDMGPU_ERR_ADMISSION.
- * Results of a request do not depend on what else is in flight. */
+ * asynchronously.  Each stream holds at most two batches in flight: under
+ * load, arrivals wait on the host and join larger batches.  Problems of at
+ * least 2^22 points run on the first stream, smaller ones on the others
+ * (nstreams > 1).  A request that must wait (no stream of its size class
+ * free, or its scratch does not fit yet) can be passed by later requests of
+ * other solvers; while it waits for scratch, at most 8 batches pass it before
+ * admission waits for it.  Solver tables stay resident between requests.  A
+ * request whose scratch alone exceeds the budget fails with
+ * FDMGPU_ERR_ADMISSION.  Results of a request do not depend on what else is
+ * in flight. */
 typedef struct fdmgpu_server_t* fdmgpu_server;
 int fdmgpu_server_create(fdmgpu_server* server, unsigned long long hbm_budget, int max_batch, int nstreams);
 int fdmgpu_server_submit(fdmgpu_server server, const fdmgpu_problem* problem, unsigned long long* ticket);
