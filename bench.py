@@ -24,7 +24,9 @@ Legs (each can be skipped):
   pub   periodic LaplCube 32^3 / 64^3 / 128^3 solved one at a time (FP64
         and FP32) beside the reference's published n-body Poisson-stage
         timings (test/nbody_result.org).
-  cpu   the reference's LaplCube<double> on the host cores (bounded sample).
+  cpu   the CPU restatement of LaplCube<double> (oracle/fdm_oracle.cpp) on the
+        host cores, OMP_PROC_BIND=close (bounded sample); the reference's own
+        build beside it when present.
 
 Prints ONE JSON line.
 """
@@ -33,8 +35,15 @@ import json
 import math
 import os
 import statistics
+import subprocess
 import sys
 import time
+
+# the CPU baseline's OpenMP threads stay on neighbouring cores (set before the
+# oracle library, and with it the OpenMP runtime, is loaded)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+# CPUs this process may run on (read before OpenMP binds the main thread to one)
+CPUS_AT_START = len(os.sched_getaffinity(0))
 
 import numpy as np
 
@@ -703,47 +712,82 @@ def leg_c5(n: int = 127, steps: int = 1000, check_at: int = 100, cpu_check: bool
     return out
 
 
+def host_cpu_info():
+    """CPU model, physical cores and the logical CPUs this process may run on."""
+    model, sockets, cores_per_socket = None, 1, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "Model name":
+                model = val
+            elif key == "Socket(s)":
+                sockets = int(val)
+            elif key == "Core(s) per socket":
+                cores_per_socket = int(val)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return {"cpu_model": model, "physical_cores": (sockets * cores_per_socket) if cores_per_socket else None,
+            "cpus_available": CPUS_AT_START}
+
+
+def time_solves(solve, budget_s: float):
+    """1 warm-up, then >= 20 timed solves (up to budget_s, at most 200)."""
+    solve()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 20 or (time.perf_counter() - t_start < budget_s and len(times) < 200):
+        t = time.perf_counter()
+        solve()
+        times.append(time.perf_counter() - t)
+        if time.perf_counter() - t_start > 4 * budget_s:
+            break
+    return times
+
+
 def cpu_baseline(n: int, f0: np.ndarray, budget_s: float = 15.0):
-    """The reference's own LaplCube<double,false> (oracle/_ref, built from the reference
-    sources) when present, else the oracle restatement; OpenMP over the host cores."""
+    """The CPU restatement of LaplCube<double,false> (oracle/fdm_oracle.cpp, the
+    reference's pass order, factors and transforms), OpenMP over this process's
+    host cores with OMP_PROC_BIND=close; when the reference's own build
+    (oracle/_ref) is present, a short sample of it is timed beside it."""
     import oracle as O
     h = 2 * math.pi / n
     l = (n + 1) * h
-    kind = "reference" if O.ref_available() else "port"
     threads = O.lib().orc_num_threads()
-    times = []
-    ans = np.zeros_like(f0)
-    if kind == "reference":
-        R = O.ref()
-        hd = R.ref_cube_create_d(h, h, h, l, l, l, n, n, n)
-        rhs = f0.copy()
-        R.ref_cube_solve_d(hd, ans, rhs)   # warm-up
-        t_start = time.perf_counter()
-        while len(times) < 20 or (time.perf_counter() - t_start < budget_s and len(times) < 200):
-            rhs = f0.copy()
-            t = time.perf_counter()
-            R.ref_cube_solve_d(hd, ans, rhs)
-            times.append(time.perf_counter() - t)
-            if time.perf_counter() - t_start > 4 * budget_s:
-                break
-        R.ref_cube_destroy(hd)
-    else:
-        O.lapl_cube(f0, h, h, h, l, l, l)
-        t_start = time.perf_counter()
-        while len(times) < 20 or (time.perf_counter() - t_start < budget_s and len(times) < 200):
-            t = time.perf_counter()
-            ans = O.lapl_cube(f0, h, h, h, l, l, l)
-            times.append(time.perf_counter() - t)
-            if time.perf_counter() - t_start > 4 * budget_s:
-                break
+    ans = None
+
+    def port():
+        nonlocal ans
+        ans = O.lapl_cube(f0, h, h, h, l, l, l)
+
+    times = time_solves(port, budget_s)
     score = unixbench_score(times)
     med = statistics.median(times)
-    return {
-        "value": 1.0 / score, "unit": "solves/s", "cores": threads, "kind": kind,
-        "sample": f"{len(times)} LaplCube<double> Dirichlet {n}^3 solves after 1 warm-up, one at a time; "
-                  f"value = 1/unixbench_score (geomean of fastest 2/3); median {1.0 / med:.2f} solves/s",
+    info = host_cpu_info()
+    out = {
+        "value": 1.0 / score, "unit": "solves/s", "cores": threads, "kind": "port",
+        "sample": f"{len(times)} LaplCube<double> Dirichlet {n}^3 solves of the CPU restatement after 1 warm-up, "
+                  f"one at a time, {threads} OpenMP threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}) on "
+                  f"{info['cpu_model']} ({info['physical_cores']} physical cores, {info['cpus_available']} CPUs "
+                  f"available to the process); value = 1/unixbench_score (geomean of fastest 2/3); median "
+                  f"{1.0 / med:.2f} solves/s",
         "median_solves_per_s": 1.0 / med,
-    }, ans
+        "host": info, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+    }
+    if O.ref_available():   # the reference's own LaplCube<double> on the same cores
+        R = O.ref()
+        hd = R.ref_cube_create_d(h, h, h, l, l, l, n, n, n)
+        ra = np.zeros_like(f0)
+
+        def ref():
+            R.ref_cube_solve_d(hd, ra, f0.copy())
+
+        rt = time_solves(ref, min(5.0, budget_s))
+        R.ref_cube_destroy(hd)
+        out["reference_solves_per_s"] = 1.0 / unixbench_score(rt)
+        out["reference_sample"] = f"{len(rt)} solves of the reference's own build (oracle/_ref), same cores"
+    return out, ans
 
 
 def main():
@@ -880,9 +924,12 @@ def main():
         progress("c5: NSCube 127^3 cavity, 1000 steps (CPU oracle check at step 100)")
         out["nscube"] = leg_c5(cpu_check="cpu" in legs)
     if "cpu" in legs:
-        progress("cpu: reference LaplCube<double> on the host cores")
+        progress("cpu: CPU restatement of LaplCube<double> on the host cores")
         cb, ans_cpu = cpu_baseline(n, fcpu, args.cpu_budget)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_host"] = cb["host"]
+        if "reference_solves_per_s" in cb:
+            out["cpu_reference_solves_per_s"] = round(cb["reference_solves_per_s"], 2)
         out["err_fp64_gpu_vs_cpu"] = float(np.linalg.norm(res_d["u0"] - ans_cpu) / np.linalg.norm(ans_cpu))
         out["err_refine_vs_cpu"] = float(np.linalg.norm(res_r["u0"] - ans_cpu) / np.linalg.norm(ans_cpu))
     print(json.dumps(out), flush=True)
