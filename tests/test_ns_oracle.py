@@ -3,9 +3,14 @@ the reference's ns_cube.cpp needs plplot and cannot be built here).  Sizes
 have n + 1 a power of two (the Poisson solve's transform length).
 
 Pins: a closed form of the first step from rest, rest staying at rest,
-discrete incompressibility of the interior after every projection, and an
-independent array-slicing transcription of init_bound / FGH / the Poisson RHS
-/ update_uvwp (src/ns_cube.cpp:64-277) compared with the oracle's loops.
+discrete incompressibility of the interior after every projection, closed
+forms of FGH (src/ns_cube.cpp:126-200) -- affine velocity fields, for which
+every averaged product term is exact (all nine advective terms), and fields
+quadratic in one coordinate (all nine viscous terms) -- and an independent
+array-slicing transcription of init_bound / FGH / the Poisson RHS /
+update_uvwp (src/ns_cube.cpp:64-277) compared with the oracle's loops.  The
+FGH closed forms fail when an index, a spacing or a term of the oracle's FGH
+is changed (checked by planting such changes in a copy).
 """
 import numpy as np
 import pytest
@@ -155,3 +160,114 @@ def test_ns_step_vs_slicing_transcription():
     for f in ("F", "G", "H", "RHS", "x", "u", "v", "w", "p"):
         got, want = ns.get(f), ref[f]
         assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max()), f
+
+
+# ---------------------------------------------------------------------------
+# FGH (src/ns_cube.cpp:126-200) pinned by closed forms.  On the staggered grid
+# u[i][k][j] sits at (x, y, z) = (j dx, (k - 1/2) dy, (i - 1/2) dz), v[i][k][j]
+# at ((j - 1/2) dx, k dy, (i - 1/2) dz), w[i][k][j] at ((j - 1/2) dx,
+# (k - 1/2) dy, i dz).  For velocity fields affine in (x, y, z) the two-point
+# averages of FGH are the fields at the midpoints and every difference of a
+# product is a central difference of a quadratic, so the discrete update is
+# exact: F = u + dt (-d(u^2)/dx - d(uv)/dy - d(uw)/dz), and likewise G, H (the
+# viscous terms vanish).  For a field quadratic in one coordinate and the
+# other components zero, only the viscous term along that coordinate is left:
+# F = u + dt/Re d2u/dy2 for u = a y^2, etc.  Only points whose stencils stay
+# clear of the boundary rows that init_bound rewrites are compared.
+# ---------------------------------------------------------------------------
+NS_GEOM = dict(x1=-np.pi, x2=np.pi, y1=-2.0, y2=2.0, z1=-1.0, z2=1.5)   # dx, dy, dz all different
+
+
+def staggered_coords(ns, which):
+    """Physical coordinates (z, y, x arrays) of every entry of u, v or w in its
+    array layout (u: j from -1, v: k from -1, w: i from -1)."""
+    nx, ny, nz = ns.nx, ns.ny, ns.nz
+    shp = ns.shapes[which]
+    ia, ka, ja = np.meshgrid(np.arange(shp[0]), np.arange(shp[1]), np.arange(shp[2]), indexing="ij")
+    i, k, j = ia.astype(float), ka.astype(float), ja.astype(float)
+    if which == "u":
+        j = j - 1
+        return (i - 0.5) * ns.dz, (k - 0.5) * ns.dy, j * ns.dx
+    if which == "v":
+        k = k - 1
+        return (i - 0.5) * ns.dz, k * ns.dy, (j - 0.5) * ns.dx
+    i = i - 1
+    return i * ns.dz, (k - 0.5) * ns.dy, (j - 0.5) * ns.dx
+
+
+def run_fgh(ns, fields):
+    for name, f in fields.items():
+        Z, Y, X = staggered_coords(ns, name)
+        ns.set(name, f(X, Y, Z))
+    ns.step(1)
+    return ns.get("F"), ns.get("G"), ns.get("H")
+
+
+def deep(a, lo=2, hi=2):
+    return a[lo:a.shape[0] - hi, lo:a.shape[1] - hi, lo:a.shape[2] - hi]
+
+
+def test_ns_fgh_affine_advection_closed_form():
+    Re, dt = 50.0, 0.02
+    ns = O.NsCube(15, 15, 7, u0=0.0, Re=Re, dt=dt, **NS_GEOM)
+    a = (0.3, 0.7, -0.4, 0.9)     # u = a0 + ax x + ay y + az z
+    b = (-0.2, 0.5, 0.8, -0.6)    # v
+    c = (0.1, -0.9, 0.35, 0.45)   # w
+
+    def aff(q):
+        return lambda X, Y, Z: q[0] + q[1] * X + q[2] * Y + q[3] * Z
+
+    F, G, H = run_fgh(ns, {"u": aff(a), "v": aff(b), "w": aff(c)})
+    nx, ny, nz = ns.nx, ns.ny, ns.nz
+    # F[i][k][j] (i 1..nz, k 1..ny, j 0..nx) at array [i-1, k-1, j], at the u point
+    i, k, j = np.meshgrid(np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(0, nx + 1), indexing="ij")
+    X, Y, Z = j * ns.dx, (k - 0.5) * ns.dy, (i - 0.5) * ns.dz
+    u, v, w = aff(a)(X, Y, Z), aff(b)(X, Y, Z), aff(c)(X, Y, Z)
+    wantF = u + dt * (-2 * u * a[1] - (a[2] * v + u * b[2]) - (a[3] * w + u * c[3]))
+    # G[i][k][j] (i 1..nz, k 0..ny, j 1..nx) at [i-1, k, j-1], at the v point
+    i, k, j = np.meshgrid(np.arange(1, nz + 1), np.arange(0, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    X, Y, Z = (j - 0.5) * ns.dx, k * ns.dy, (i - 0.5) * ns.dz
+    u, v, w = aff(a)(X, Y, Z), aff(b)(X, Y, Z), aff(c)(X, Y, Z)
+    wantG = v + dt * (-2 * v * b[2] - (a[1] * v + u * b[1]) - (b[3] * w + v * c[3]))
+    # H[i][k][j] (i 0..nz, k 1..ny, j 1..nx) at [i, k-1, j-1], at the w point
+    i, k, j = np.meshgrid(np.arange(0, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    X, Y, Z = (j - 0.5) * ns.dx, (k - 0.5) * ns.dy, i * ns.dz
+    u, v, w = aff(a)(X, Y, Z), aff(b)(X, Y, Z), aff(c)(X, Y, Z)
+    wantH = w + dt * (-2 * w * c[3] - (a[1] * w + u * c[1]) - (b[2] * w + v * c[2]))
+    for got, want in ((F, wantF), (G, wantG), (H, wantH)):
+        assert got.shape == want.shape
+        scale = np.abs(want).max()
+        assert np.abs(deep(got) - deep(want)).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("comp", ["u", "v", "w"])
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_ns_fgh_viscous_closed_form(comp, axis):
+    """One component q s^2 (s = x, y or z), the others 0.  The advective terms
+    are even in q and the viscous ones odd, so half the difference of the runs
+    with q and -q is u + dt/Re d2u/ds2 = q s^2 + 2 q dt / Re, for every one of
+    the nine (component, axis) viscous terms."""
+    Re, dt, q = 7.0, 0.01, 0.8
+    got = []
+    for sgn in (1.0, -1.0):
+        ns = O.NsCube(15, 15, 7, u0=0.0, Re=Re, dt=dt, **NS_GEOM)
+        fields = {"u": lambda X, Y, Z: 0 * X, "v": lambda X, Y, Z: 0 * X, "w": lambda X, Y, Z: 0 * X}
+        ax = {"x": 0, "y": 1, "z": 2}[axis]
+        fields[comp] = lambda X, Y, Z, ax=ax, sgn=sgn: sgn * q * (X, Y, Z)[ax] ** 2
+        F, G, H = run_fgh(ns, fields)
+        got.append({"u": F, "v": G, "w": H}[comp])
+        for other in {"u": (G, H), "v": (F, H), "w": (F, G)}[comp]:
+            assert np.abs(deep(other)).max() == 0.0
+    odd = 0.5 * (got[0] - got[1])
+    nx, ny, nz = ns.nx, ns.ny, ns.nz
+    if comp == "u":
+        i, k, j = np.meshgrid(np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(0, nx + 1), indexing="ij")
+        X, Y, Z = j * ns.dx, (k - 0.5) * ns.dy, (i - 0.5) * ns.dz
+    elif comp == "v":
+        i, k, j = np.meshgrid(np.arange(1, nz + 1), np.arange(0, ny + 1), np.arange(1, nx + 1), indexing="ij")
+        X, Y, Z = (j - 0.5) * ns.dx, k * ns.dy, (i - 0.5) * ns.dz
+    else:
+        i, k, j = np.meshgrid(np.arange(0, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+        X, Y, Z = (j - 0.5) * ns.dx, (k - 0.5) * ns.dy, i * ns.dz
+    want = q * (X, Y, Z)[{"x": 0, "y": 1, "z": 2}[axis]] ** 2 + dt * 2 * q / Re
+    assert np.abs(deep(odd) - deep(want)).max() <= 1e-12 * np.abs(want).max()
