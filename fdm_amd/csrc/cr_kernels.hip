@@ -544,6 +544,73 @@ __global__ __launch_bounds__(256) void rowcr_kernel(const BatchPtr<Ts> data, con
     for (int j = threadIdx.x; j < nx; j += blockDim.x) row[j] = (Ts)bb[j];
 }
 
+// ---------------------------------------------------------------------------
+// LaplRect rows that are not diagonally dominant (lm_y_scale / L_scale /
+// U_scale chosen so): the reference solves every row with LAPACK ?gtsv
+// (src/lapl_rect.cpp:90, src/blas.h:67-75), Gaussian elimination with partial
+// pivoting -- a row interchange when |d_i| < |dl_i| -- and back substitution
+// over the second superdiagonal the interchanges fill in (LAPACK's published
+// algorithm).  Elimination is sequential along a row: one block per row, the
+// coefficients and the right-hand side staged in LDS by all threads, the
+// elimination by one thread.  Row systems are those of rowcr_kernel:
+// dl_j = L_scale(j+1)/dx^2 (row j+1's sub-diagonal), d_j, du_j = U_scale(j)/dx^2.
+// Singular rows (a zero pivot, where the reference aborts) are rejected on the
+// host when the scales are set.
+// ---------------------------------------------------------------------------
+template <typename Ts, typename T>
+__global__ __launch_bounds__(64) void rowgtsv_kernel(const BatchPtr<Ts> data, const int nx,
+                                                     const Ts* __restrict__ lmy, const Ts* __restrict__ sc,
+                                                     const Ts* __restrict__ Ls, const Ts* __restrict__ Us,
+                                                     const double inv_dx2, const int* __restrict__ done,
+                                                     const Ts* __restrict__ lmz, const int rdiv) {
+    if (done && done[blockIdx.y]) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* dl = reinterpret_cast<T*>(smem_raw);
+    T* d = dl + nx;
+    T* du = d + nx;
+    T* b = du + nx;
+    const int k = blockIdx.x;
+    const auto row = as_global(data(blockIdx.y) + (long)k * nx);
+    const Ts lk = lmy[lmz ? k / rdiv : k];
+    const double lz = lmz ? (double)lmz[k % rdiv] : 0.0;
+    for (int j = threadIdx.x; j < nx; j += blockDim.x) {
+        d[j] = (T)(-2.0 * inv_dx2 - (double)(lk * sc[j]) - lz);
+        dl[j] = j + 1 < nx ? (T)(Ls[j + 1] * inv_dx2) : T(0);
+        du[j] = j + 1 < nx ? (T)(Us[j] * inv_dx2) : T(0);
+        b[j] = (T)row[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int n = nx;
+        for (int i = 0; i < n - 1; i++) {
+            if (fabs(d[i]) >= fabs(dl[i])) {   // no interchange
+                const T fact = dl[i] / d[i];
+                d[i + 1] -= fact * du[i];
+                b[i + 1] -= fact * b[i];
+                if (i < n - 2) dl[i] = T(0);
+            } else {                           // interchange rows i and i+1
+                const T fact = d[i] / dl[i];
+                d[i] = dl[i];
+                T temp = d[i + 1];
+                d[i + 1] = du[i] - fact * temp;
+                if (i < n - 2) {
+                    dl[i] = du[i + 1];         // second superdiagonal
+                    du[i + 1] = -fact * dl[i];
+                }
+                du[i] = temp;
+                temp = b[i];
+                b[i] = b[i + 1];
+                b[i + 1] = temp - fact * b[i + 1];
+            }
+        }
+        b[n - 1] /= d[n - 1];
+        if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / d[n - 2];
+        for (int i = n - 3; i >= 0; i--) b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nx; j += blockDim.x) row[j] = (Ts)b[j];
+}
+
 // Lane multipliers a z system of 2^q rows supports: 1 always, 2 when every
 // lane still holds at least 2 rows; the strip layouts (3: 64 systems per
 // block, 4: 128) from 32 rows up (Dirichlet only), 4 up to 128 rows (512
@@ -649,8 +716,26 @@ static hipError_t rowcr_k(const RowCrArgs& a) {
     return hipGetLastError();
 }
 
+template <typename Ts, typename T>
+static hipError_t rowgtsv_k(const RowCrArgs& a) {
+    const size_t lds = 4 * sizeof(T) * (size_t)a.nx;
+    hipError_t e = allow_lds(rowgtsv_kernel<Ts, T>, lds);
+    if (e != hipSuccess) return e;
+    const BatchPtr<Ts> data{(Ts*)a.data, a.data_tab, a.bs};
+    hipLaunchKernelGGL((rowgtsv_kernel<Ts, T>), dim3(a.nrows, a.batch), dim3(64), lds, a.stream, data, a.nx,
+                       (const Ts*)a.lmy, (const Ts*)a.sc, (const Ts*)a.Ls, (const Ts*)a.Us, a.inv_dx2, a.done,
+                       (const Ts*)a.lmz, a.rdiv);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_rowcr(const RowCrArgs& a) {
+    if (a.pivot) {
+        if constexpr (std::is_same_v<T, float>) {
+            if (a.wide) return rowgtsv_k<float, double>(a);
+        }
+        return rowgtsv_k<T, T>(a);
+    }
     if constexpr (std::is_same_v<T, float>) {
         if (a.wide) return rowcr_k<float, double>(a);
     }

@@ -475,6 +475,7 @@ int stage_tridiag(fdmgpu_plan_t* P, void* data, Tab data_tab, int count, hipStre
     }
     RowCrArgs r{data, P->nx, P->ny, P->lmy, P->d_sc, P->d_Ls, P->d_Us, 1.0 / (P->dx * P->dx), elems, count, s, done,
                 P->precision == FDMGPU_F32, data_tab};
+    r.pivot = P->pivot_rows;
     Timed tm(P, FDMGPU_K_ROWCR, s);
     HIPCHK(launch_rowcr<T>(r));
     return FDMGPU_OK;
@@ -712,6 +713,26 @@ int set_scales_fft2(fdmgpu_plan_t* P, const double* s) {
     return upload<double>((void**)&P->d_sc64, P->h_sc);
 }
 
+// The pivot sequence of ?gtsv on one tridiagonal system (coefficients only,
+// overwritten): the 1-based column of the first zero pivot, 0 if none.
+int gtsv_zero_pivot(const int n, double* dl, double* d, double* du) {
+    for (int i = 0; i < n - 1; i++) {
+        if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+            if (d[i] == 0.0) return i + 1;
+            const double fact = dl[i] / d[i];
+            d[i + 1] -= fact * du[i];
+        } else {
+            const double fact = d[i] / dl[i];
+            d[i] = dl[i];
+            const double temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            if (i < n - 2) du[i + 1] = -fact * du[i + 1];
+            du[i] = temp;
+        }
+    }
+    return n > 0 && d[n - 1] == 0.0 ? n : 0;
+}
+
 int set_scales_impl(fdmgpu_plan_t* P, const double* s, const double* L, const double* U) {
     if (P->fft2) return set_scales_fft2(P, s);
     const int nx = P->nx;
@@ -730,23 +751,42 @@ int set_scales_impl(fdmgpu_plan_t* P, const double* s, const double* L, const do
             P->h_Us[j] = (double)(float)P->h_Us[j];
         }
     }
-    // the GPU eliminates without row interchanges: require diagonal dominance of
-    // every row system (the default scales and the cylindrical ones satisfy it)
+    // Row systems that are all diagonally dominant (the default scales, the
+    // cylindrical ones) are solved by cyclic reduction; otherwise every row
+    // goes through Gaussian elimination with partial pivoting, the reference's
+    // ?gtsv.  A row whose elimination meets a zero pivot is singular: the
+    // reference aborts there (verify(info == 0), src/lapl_rect.cpp:91).
     std::vector<double> lam = P->periodic ? periodic_lambda(P->ny, P->dy, false) : dirichlet_lambda(P->ny, P->dy, false);
     const double i2 = 1.0 / (P->dx * P->dx);
-    for (int k = 0; k < P->ny; k++) {
+    bool pivot = false, strict = true;   // some row not dominant / every row strictly dominant
+    for (int k = 0; k < P->ny && !pivot; k++) {
         for (int j = 0; j < nx; j++) {
             const double d = std::fabs(-2.0 * i2 - lam[k] * P->h_sc[j]);
             const double off = (j > 0 ? std::fabs(P->h_Ls[j]) : 0.0) * i2 + (j < nx - 1 ? std::fabs(P->h_Us[j]) : 0.0) * i2;
+            if (!(d > off)) strict = false;
             if (!(d >= off) || d == 0.0) {
+                pivot = true;
+                break;
+            }
+        }
+    }
+    if (!strict) {   // not strictly dominant: a row may be singular
+        std::vector<double> dl(nx), d(nx), du(nx);
+        for (int k = 0; k < P->ny; k++) {
+            for (int j = 0; j < nx; j++) {
+                d[j] = -2.0 * i2 - lam[k] * P->h_sc[j];
+                dl[j] = j + 1 < nx ? P->h_Ls[j + 1] * i2 : 0.0;
+                du[j] = j + 1 < nx ? P->h_Us[j] * i2 : 0.0;
+            }
+            const int info = gtsv_zero_pivot(nx, dl.data(), d.data(), du.data());
+            if (info) {
                 std::snprintf(g_errbuf, sizeof g_errbuf,
-                              "LaplRect row %d column %d is not diagonally dominant: the GPU tridiagonal solve "
-                              "does not pivot",
-                              k + 1, j + 1);
+                              "LaplRect row %d is singular (gtsv info %d: zero pivot in column %d)", k + 1, info, info);
                 return FDMGPU_ERR_SINGULAR;
             }
         }
     }
+    P->pivot_rows = pivot;
     const bool f32 = P->precision != FDMGPU_F64;
     int rc;
     if ((rc = f32 ? upload<float>(&P->d_sc, P->h_sc) : upload<double>(&P->d_sc, P->h_sc))) return rc;

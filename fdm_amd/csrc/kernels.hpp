@@ -154,6 +154,8 @@ struct RowCrArgs {
     const void* const* data_tab;
     const void* lmz = nullptr;   // LaplCyl3FFT2: second per-row eigenvalue, row k uses lmy[k / rdiv] and
     int rdiv = 1;                //   lmz[k % rdiv] (D_j = -2/dx^2 - lmy*s_j - lmz)
+    bool pivot = false;          // Gaussian elimination with partial pivoting (rows that are not
+                                 //   diagonally dominant) instead of cyclic reduction
 };
 
 // LaplRectFFT2 spectral solve: c[k][j] /= -(lmy[k] * sc[j] + lmx[j]) in the

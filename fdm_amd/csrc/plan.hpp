@@ -72,6 +72,7 @@ struct fdmgpu_plan_t {
     double* d_Ls64 = nullptr;
     double* d_Us64 = nullptr;
     bool scaled = false;
+    bool pivot_rows = false;            // LaplRect rows not all diagonally dominant: ?gtsv with pivoting
     // refinement
     double rtol = 1e-10;
     int max_iter = 4;

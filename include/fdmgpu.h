@@ -35,7 +35,7 @@ typedef struct fdmgpu_plan_t* fdmgpu_plan;
 #define FDMGPU_ERR_UNSUPPORTED 3   /* valid for the reference, not implemented on the GPU */
 #define FDMGPU_ERR_OUT_OF_MEMORY 4 /* device allocation failed */
 #define FDMGPU_ERR_DEVICE 5        /* HIP runtime error (no device, launch failure) */
-#define FDMGPU_ERR_SINGULAR 6      /* tridiagonal system not solvable without pivoting: gtsv info != 0, src/lapl_rect.cpp:91 */
+#define FDMGPU_ERR_SINGULAR 6      /* singular LaplRect row system: gtsv info != 0, src/lapl_rect.cpp:91 */
 #define FDMGPU_ERR_NOT_CONVERGED 7 /* FP32+refine did not reach the requested residual */
 #define FDMGPU_ERR_ADMISSION 8     /* serving: request larger than the HBM budget */
 #define FDMGPU_ERR_TOO_LARGE 9     /* fdmgpu_solve_batch_noalloc: scratch beyond the static scratch array */
@@ -93,9 +93,12 @@ int fdmgpu_plan_cyl3_fft2(fdmgpu_plan* plan, int precision, int periodic_mask, d
 
 /* Replaces the public LaplRect members lm_y_scale, L_scale, U_scale
  * (src/lapl_rect.h:32, 57-59): arrays of nx+1 doubles indexed 1..nx (entry 0
- * unused, as in the reference); NULL keeps the default of all ones.  Returns
- * FDMGPU_ERR_SINGULAR when a row would need pivoting (the GPU eliminates
- * without row interchanges; the reference's gtsv pivots). */
+ * unused, as in the reference); NULL keeps the default of all ones.  Row
+ * systems that are all diagonally dominant are solved by cyclic reduction;
+ * otherwise every row by Gaussian elimination with partial pivoting, as the
+ * reference's ?gtsv (src/lapl_rect.cpp:90).  Returns FDMGPU_ERR_SINGULAR when
+ * a row system meets a zero pivot (gtsv info != 0, where the reference
+ * aborts, src/lapl_rect.cpp:91). */
 int fdmgpu_rect_set_scales(fdmgpu_plan plan, const double* lm_y_scale, const double* L_scale,
                            const double* U_scale);
 

@@ -334,10 +334,42 @@ def test_errors(fdm):
     with pytest.raises(fdm.FdmGpuError) as e:
         fdm.LaplRect(0.1, 0.1, 1, 1, 10, 10)                  # ny+1 = 11
     assert e.value.code == fdm.ERR_INVALID_SIZE
-    s = fdm.LaplRect(0.1, 0.1, 1.1, 3.2, 10, 31)
-    with pytest.raises(fdm.FdmGpuError) as e:                 # would need pivoting
-        s.set_scales(np.zeros(11), np.full(11, 5.0), np.full(11, 5.0))
+    s = fdm.LaplRect(0.1, 0.1, 0.3, 3.2, 2, 31)
+    with pytest.raises(fdm.FdmGpuError) as e:                 # every row [[d, u], [l, d]] with d*d = u*l
+        s.set_scales(np.zeros(3), np.full(3, 2.0), np.full(3, 2.0))
     assert e.value.code == fdm.ERR_SINGULAR
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "refine"])
+def test_rect_rows_needing_pivoting(fdm, prec):
+    """Scales that break diagonal dominance, so that gtsv interchanges rows in
+    29 of the 31 row systems (the reference's gtsv pivots, src/lapl_rect.cpp:90):
+    the GPU's pivoting elimination against the oracle's LaplRect (dgtsv
+    restated).  The row systems' condition numbers reach 372, which scales the
+    bars: FP64 within 1e-12, FP32 within 1e-4 of the FP64 oracle (372 x float
+    rounding), FP32+refine to a residual of 1e-10 and within 1e-9."""
+    nx, ny = 100, 31
+    say("rect pivoting", prec, nx, ny)
+    h = 0.1
+    g = (h, h, (nx + 1) * h, (ny + 1) * h)
+    j = np.arange(nx + 1)
+    lm = 0.01 + 0.08 * np.sin(0.37 * j)         # |d| = 2/h^2 + lam * s below |L| + |U| = 3.8/h^2
+    Ls = 1.91 + 0.085 * np.cos(0.21 * j)
+    Us = 1.92 + 0.085 * np.sin(0.13 * j)
+    rhs = O.field(77, (ny, nx))
+    ref = O.lapl_rect(rhs, *g, lm_y_scale=lm, L_scale=Ls, U_scale=Us)
+    s = fdm.LaplRect(*g, nx, ny, precision=prec)
+    s.set_scales(lm, Ls, Us)
+    if prec == "f32":
+        ans = np.zeros((ny, nx), np.float32)
+        s.solve(ans, rhs.astype(np.float32))
+        assert rel2(ans, ref) <= 1e-4
+    else:
+        ans = np.zeros_like(rhs)
+        s.solve(ans, rhs)
+        assert rel2(ans, ref) <= (1e-12 if prec == "f64" else 1e-9)
+        if prec == "refine":
+            assert 0 <= s.stats().rel_residual <= 1e-10
 
 
 @pytest.mark.parametrize("periodic", [False, True])
