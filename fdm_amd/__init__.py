@@ -39,7 +39,7 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdm
            "fdmgpu_server_submit", "fdmgpu_server_set_refine", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
            "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_tune_set", "fdmgpu_ns_create", "fdmgpu_ns_step", "fdmgpu_ns_get", "fdmgpu_ns_set", "fdmgpu_ns_info",
            "fdmgpu_ns_solver", "fdmgpu_ns_destroy", "fdmgpu_destroy", "fdmgpu_strerror",
-           "fdmgpu_version", "fdmgpu_solve_batch_noalloc")
+           "fdmgpu_version", "fdmgpu_solve_batch_noalloc", "fdmgpu_group_set_refine")
 
 # kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
 K_XY_FWD, K_XY_INV, K_X_TILE, K_Y_TILE, K_ZCR, K_ROWCR, K_RESIDUAL, K_OTHER = range(8)
@@ -125,6 +125,7 @@ def _lib_handle():
         ull = C.c_ulonglong
         L.fdmgpu_group_create.argtypes = [C.POINTER(vp), i, C.POINTER(Problem), i, i]
         L.fdmgpu_group_solve.argtypes = [vp, vp]
+        L.fdmgpu_group_set_refine.argtypes = [vp, d, i]
         L.fdmgpu_group_result.argtypes = [vp, i, C.POINTER(Result)]
         L.fdmgpu_group_destroy.argtypes = [vp]
         L.fdmgpu_server_create.argtypes = [C.POINTER(vp), ull, i, i]

@@ -200,16 +200,21 @@ int work_reserve(const fdmgpu_plan_t* P, Work& W, int count) {
 }
 
 bool plane_capable(const fdmgpu_plan_t* P) {
-    if (P->kind != 3 || P->logx != P->logy) return false;
+    if (P->kind != 3) return false;
+    if (P->logx != P->logy)   // rectangular Dirichlet planes: plane_rect_kernel
+        return P->periodic == 0 && plane_rect_fits(P->logx, P->logy, P->precision == FDMGPU_F64);
     return P->precision == FDMGPU_F64 ? plane_fits<double>(P->logx) : plane_fits<float>(P->logx);
 }
+
+// Square planes (the plane kernels with the fused residual and the streaming variant).
+static bool square_planes(const fdmgpu_plan_t* P) { return P->logx == P->logy; }
 
 // Plane kernel of the default variant: the streaming kernel for plain FP64
 // planes of N = 128 (one plane fills a CU's LDS), else the plane kernel where
 // the plane fits LDS, else the axis tiles.
 int default_plane(const fdmgpu_plan_t* P) {
     if (!plane_capable(P)) return 0;
-    return plane_stream_ok(P->logx, P->precision == FDMGPU_F64) ? 2 : 1;
+    return square_planes(P) && plane_stream_ok(P->logx, P->precision == FDMGPU_F64) ? 2 : 1;
 }
 
 std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
@@ -231,7 +236,7 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
     out.push_back(d);   // the default first
     const bool refine = P->precision == FDMGPU_F32_REFINE64;
     for (int gm : gms) {
-        if (plane_capable(P) && plane_stream_ok(P->logx, P->precision == FDMGPU_F64)) {
+        if (plane_capable(P) && square_planes(P) && plane_stream_ok(P->logx, P->precision == FDMGPU_F64)) {
             Variant v;   // plane_stream_kernel (plain FP64 planes of N = 128)
             v.plane = 2;
             v.zcr_gmul = gm;
@@ -240,7 +245,7 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
         if (plane_capable(P)) {
             // fuse selects the refinement's fused residual pass: plain plans
             // list 0 only, the value of their default
-            for (int fu : (refine ? std::vector<int>{1, 0} : std::vector<int>{0})) {
+            for (int fu : (refine && square_planes(P) ? std::vector<int>{1, 0} : std::vector<int>{0})) {
                 Variant v;
                 v.plane = 1;
                 v.zcr_gmul = gm;
@@ -349,6 +354,14 @@ int stage_fwd(fdmgpu_plan_t* P, const void* in, Tab in_tab, void* out, Tab out_t
         HIPCHK((launch_plane<Tin, T, T>(P->logx, a)));
         return FDMGPU_OK;
     }
+    if (P->kind == 3 && P->var.plane && P->logx != P->logy && P->kfwd == kDst &&
+        plane_rect_fits(P->logx, P->logy, sizeof(T) == 8)) {
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, false, done, P->kfwd, in_tab, out_tab};
+        a.tables_y = P->tab_y;
+        Timed tm(P, plane_class, s);
+        HIPCHK((launch_plane_rect<Tin, T, T>(P->logx, P->logy, a)));
+        return FDMGPU_OK;
+    }
     if (P->kind == 3) {
         AxisArgs ax{in, out, (long)P->ny * P->nz, P->nx, P->nz, elems, elems, count, sx, P->tab_x, s, false, done,
                     P->kfwd, in_tab, out_tab, P->var.tile_x};
@@ -405,6 +418,15 @@ int stage_inv(fdmgpu_plan_t* P, void* in, Tab in_tab, void* out, Tab out_tab, in
         a.stream_planes = P->var.plane == 2;
         Timed tm(P, accum ? FDMGPU_K_XY_INV_ACC : FDMGPU_K_XY_INV, s);
         HIPCHK((launch_plane<T, Tout, T>(P->logx, a)));
+        return FDMGPU_OK;
+    }
+    if (P->kind == 3 && P->var.plane && P->logx != P->logy && P->kinv == kDst &&
+        plane_rect_fits(P->logx, P->logy, sizeof(T) == 8)) {
+        PlaneArgs a{in, out, elems, elems, P->nz, count, sx, sy, P->tab_x, s, accum, done, P->kinv, in_tab, out_tab,
+                    base};
+        a.tables_y = P->tab_y;
+        Timed tm(P, accum ? FDMGPU_K_XY_INV_ACC : FDMGPU_K_XY_INV, s);
+        HIPCHK((launch_plane_rect<T, Tout, T>(P->logx, P->logy, a)));
         return FDMGPU_OK;
     }
     if (base) return FDMGPU_ERR_INVALID_ARG;   // only the plane path adds onto a separate base
@@ -556,7 +578,7 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
     ra.stream = s;
     ra.f_tab = rhs_tab;
     ra.u_tab = ans_tab;
-    const bool fused = plane_capable(P) && P->var.plane && P->var.fuse;
+    const bool fused = plane_capable(P) && square_planes(P) && P->var.plane && P->var.fuse;
     // u1 in FP32 scratch: separate-residual plane path (Dirichlet planes), solves 1 and 2 in
     // this call (the fused forward pass reads an FP32 iterate slower than it saves)
     const bool u1f = plane_capable(P) && P->var.plane && !fused && P->kinv == kDst && W.u32 && first == 1 &&

@@ -101,6 +101,7 @@ struct PlaneArgs {
     const float* base = nullptr;  // with accum: out = base + result (base contiguous, stride bs_out);
                                   // problems flagged done get out = base
     bool stream_planes = false;   // plane_stream_kernel: FP64, N = 128, plain (no accum / base / done)
+    const void* tables_y = nullptr;   // rectangular planes: the y axis's DST tables (tables: x)
 };
 
 struct AxisArgs {
@@ -200,6 +201,11 @@ template <typename T>
 bool plane_fits(int logn);
 // plane_stream_kernel exists for this plane (FP64 planes of N = 128)
 bool plane_stream_ok(int logn, bool f64);
+// Dirichlet planes of (2^logy - 1) x (2^logx - 1) points, logx != logy, both
+// 4 .. 7: the rectangular plane kernel (plane_rect_kernels.hip)
+template <typename Tin, typename Tout, typename T>
+hipError_t launch_plane_rect(int logx, int logy, const PlaneArgs& a);
+bool plane_rect_fits(int logx, int logy, bool f64);
 template <typename Tin, typename Tout, typename T>
 hipError_t launch_xaxis(int logn, const AxisArgs& a);
 template <typename Tin, typename Tout, typename T>

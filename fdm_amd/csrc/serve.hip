@@ -323,6 +323,23 @@ int fdmgpu_group_solve(fdmgpu_group g, void* stream) {
     return FDMGPU_OK;
 }
 
+int fdmgpu_group_set_refine(fdmgpu_group g, double rel_tol, int max_iter) {
+    if (!g || !(rel_tol > 0) || max_iter < 1 || max_iter > 64) return FDMGPU_ERR_INVALID_ARG;
+    SRV_HIPCHK(hipSetDevice(g->device));
+    for (hipStream_t s : g->streams) SRV_HIPCHK(hipStreamSynchronize(s));
+    for (Bucket& b : g->buckets)
+        if (b.key.precision == FDMGPU_F32_REFINE64) {
+            const int rc = fdmgpu_set_refine(b.plan, rel_tol, max_iter);
+            if (rc) return rc;
+        }
+    // the captured sequence holds the old launches: capture again
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+    return FDMGPU_OK;
+}
+
 int fdmgpu_group_result(fdmgpu_group g, int i, fdmgpu_result* r) {
     if (!g || !r || i < 0 || i >= (int)g->probs.size()) return FDMGPU_ERR_INVALID_ARG;
     SRV_HIPCHK(hipSetDevice(g->device));

@@ -50,6 +50,10 @@ class Group:
         _check(_lib_handle().fdmgpu_group_result(self._h, i, C.byref(r)), "group_result")
         return r
 
+    def set_refine(self, rel_tol: float = 1e-10, max_iter: int = 4):
+        """Refinement settings of the group's FP32+refine solvers (recaptures the graph)."""
+        _check(_lib_handle().fdmgpu_group_set_refine(self._h, rel_tol, max_iter), "group_set_refine")
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib_handle().fdmgpu_group_destroy(self._h)

@@ -246,6 +246,11 @@ typedef struct fdmgpu_group_t* fdmgpu_group;
 #define FDMGPU_GROUP_GRAPH 1
 int fdmgpu_group_create(fdmgpu_group* group, int count, const fdmgpu_problem* problems, int nstreams, int flags);
 int fdmgpu_group_solve(fdmgpu_group group, void* stream);
+/* Refinement tolerance and inner-solve limit of the group's FP32+refine
+ * solvers (defaults 1e-10, 4), as fdmgpu_set_refine per solver; synchronises
+ * the group and drops a captured graph (the next solve captures again).
+ * No reference counterpart. */
+int fdmgpu_group_set_refine(fdmgpu_group group, double rel_tol, int max_iter);
 /* After the solve completed (synchronises): result of problem i. */
 int fdmgpu_group_result(fdmgpu_group group, int i, fdmgpu_result* result);
 int fdmgpu_group_destroy(fdmgpu_group group);

@@ -149,6 +149,12 @@ def test_c3_uneven_512_group_graph(fdm, hip, srv):
             u, f = sample[i]
             assert rel2(a, u) <= 1e-9, (i, mix[i])
             assert O.rel_residual(f, a, *spacings(mix[i])) <= 1e-10, (i, mix[i])
+    # as bench.py does: exactly the inner solves the eager run needed, which
+    # changes no converged answer
+    maxit = max(grp.result(i).iterations for i in range(count))
+    grp.set_refine(1e-10, maxit)
+    grp.solve(st)
+    st.synchronize()
     for rep in range(2):     # the second solve captures the graph, the third replays it
         say("  c3 graph solve", rep + 2)
         for d_u, _ in bufs:

@@ -315,10 +315,10 @@ def _continuation_case(fdm, hip, srv, s, g, nx, ny, nz, f):
 def test_tune_set_rejects_invalid_variant(fdm, hip, srv):
     """fdmgpu_tune_set accepts only variants valid for the shape."""
     say("tune_set invalid")
-    g = geom_cube(63, 31, 15)
-    d_x, d_y = hip.DeviceArray((15, 31, 63), np.float64), hip.DeviceArray((15, 31, 63), np.float64)
-    p = srv.cube_problem(*g, 63, 31, 15, d_x, d_y)
-    with pytest.raises(fdm.FdmGpuError) as e:      # non-square planes have no plane kernel
+    g = geom_cube(255, 31, 15)
+    d_x, d_y = hip.DeviceArray((15, 31, 255), np.float64), hip.DeviceArray((15, 31, 255), np.float64)
+    p = srv.cube_problem(*g, 255, 31, 15, d_x, d_y)
+    with pytest.raises(fdm.FdmGpuError) as e:      # 255-wide rows: no plane kernel (axis tiles only)
         srv.tune_set(p, (1, 0, 0, 0, 0))
     assert e.value.code == fdm.ERR_INVALID_ARG
     with pytest.raises(fdm.FdmGpuError):           # no such tile
