@@ -38,7 +38,8 @@ EXPORTS = ("fdmgpu_plan_cube", "fdmgpu_plan_rect", "fdmgpu_plan_rect_fft2", "fdm
            "fdmgpu_group_solve", "fdmgpu_group_result", "fdmgpu_group_destroy", "fdmgpu_server_create",
            "fdmgpu_server_submit", "fdmgpu_server_set_refine", "fdmgpu_server_poll", "fdmgpu_server_wait", "fdmgpu_server_drain",
            "fdmgpu_server_forget", "fdmgpu_server_destroy", "fdmgpu_tune", "fdmgpu_tune_set", "fdmgpu_ns_create", "fdmgpu_ns_step", "fdmgpu_ns_get", "fdmgpu_ns_set", "fdmgpu_ns_info",
-           "fdmgpu_ns_solver", "fdmgpu_ns_destroy", "fdmgpu_destroy", "fdmgpu_strerror",
+           "fdmgpu_ns_step_timed", "fdmgpu_ns_maxabs", "fdmgpu_ns_solver", "fdmgpu_ns_destroy", "fdmgpu_destroy",
+           "fdmgpu_strerror",
            "fdmgpu_version", "fdmgpu_solve_batch_noalloc", "fdmgpu_group_set_refine")
 
 # kernel classes for fdmgpu_get_timing (include/fdmgpu.h)
@@ -144,6 +145,8 @@ def _lib_handle():
         L.fdmgpu_ns_set.argtypes = [vp, i, vp, ull]
         L.fdmgpu_ns_info.argtypes = [vp, i, C.POINTER(ull), C.POINTER(vp)]
         L.fdmgpu_ns_solver.argtypes = [vp, C.POINTER(vp)]
+        L.fdmgpu_ns_step_timed.argtypes = [vp, C.POINTER(C.c_double), vp]
+        L.fdmgpu_ns_maxabs.argtypes = [vp, C.POINTER(C.c_double)]
         L.fdmgpu_ns_destroy.argtypes = [vp]
         L.fdmgpu_destroy.argtypes = [vp]
         L.fdmgpu_strerror.argtypes = [i]
@@ -467,6 +470,21 @@ class NSCube:
     def step(self, count: int = 1, stream=None):
         _check(_lib_handle().fdmgpu_ns_step(self._h, count, _stream_handle(stream)), "NSCube.step")
         self.time_index += count
+
+    def step_timed(self, stream=None) -> tuple:
+        """One step, eagerly, with GPU events at its phases: (init_bound, FGH,
+        poisson, update_uvwp) in ms (reference src/ns_cube.cpp:27-50)."""
+        ms = (C.c_double * 4)()
+        _check(_lib_handle().fdmgpu_ns_step_timed(self._h, ms, _stream_handle(stream)), "NSCube.step_timed")
+        self.time_index += 1
+        return tuple(ms)
+
+    def maxabs(self) -> dict:
+        """Largest |value| of every field, ghost layers included, reduced on the
+        GPU (tensor::maxabs, reference src/tensor.h:269-273)."""
+        out = (C.c_double * 9)()
+        _check(_lib_handle().fdmgpu_ns_maxabs(self._h, out), "NSCube.maxabs")
+        return dict(zip(NS_FIELDS, out))
 
     def get(self, name: str) -> np.ndarray:
         """A copy of field `name`.  Raises FdmGpuError (ERR_NOT_CONVERGED) when

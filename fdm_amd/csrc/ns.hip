@@ -215,6 +215,48 @@ __global__ void ns_converged_kernel(const double* __restrict__ resid, const doub
     if (threadIdx.x == 0 && !(resid[0] <= tol)) bad[0] = 1;
 }
 
+// Largest |value| of each of the nine arrays, ghost layers included, as
+// tensor::maxabs (src/tensor.h:269-273): the running maximum starts at 0 and
+// takes |b| only when m < |b|, so a NaN never enters it.  Pass 1: block
+// (b, which) writes the maximum of its grid-stride share of array `which` to
+// part[which][b]; pass 2: thread `which` folds that row.
+constexpr int kMaxabsBlocks = 128, kMaxabsThreads = 256;
+
+struct FieldSet {
+    const double* a[9];
+    unsigned long long n[9];
+};
+
+__device__ __forceinline__ double max_keep(double m, double b) { return m < b ? b : m; }
+
+__global__ __launch_bounds__(kMaxabsThreads) void ns_maxabs_part_kernel(FieldSet fs, double* __restrict__ part) {
+    __shared__ double wmax[kMaxabsThreads / 64];
+    const int which = blockIdx.y;
+    const auto a = as_global(fs.a[which]);
+    const unsigned long long n = fs.n[which];
+    double m = 0.0;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * kMaxabsThreads + threadIdx.x; i < n;
+         i += (unsigned long long)kMaxabsBlocks * kMaxabsThreads)
+        m = max_keep(m, fabs(a[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max_keep(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = wmax[0];
+        for (int w = 1; w < kMaxabsThreads / 64; w++) r = max_keep(r, wmax[w]);
+        part[which * kMaxabsBlocks + blockIdx.x] = r;
+    }
+}
+
+__global__ void ns_maxabs_final_kernel(const double* __restrict__ part, double* __restrict__ out) {
+    const int which = threadIdx.x;
+    if (which >= 9) return;
+    double m = 0.0;
+    for (int b = 0; b < kMaxabsBlocks; b++) m = max_keep(m, part[which * kMaxabsBlocks + b]);
+    out[which] = m;
+}
+
 }  // namespace
 
 struct fdmgpu_ns_t {
@@ -231,6 +273,8 @@ struct fdmgpu_ns_t {
     bool warm_ok = false;          // x holds the pressure of the previous step (a field upload clears it)
     bool use_graph = true;
     int* d_bad = nullptr;          // set on the device when a projection misses its tolerance (sticky)
+    double* d_max = nullptr;       // maxabs: 9 x kMaxabsBlocks partial maxima, then the 9 results
+    hipEvent_t ev_phase[5] = {};   // timed steps: the bounds of the four phases
 };
 
 namespace {
@@ -244,10 +288,16 @@ namespace {
         }                                                                                \
     } while (0)
 
-int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s) {
+// One step on stream s.  ev (a timed step, never under capture): events
+// recorded at the bounds of the reference's four phases, init_bound (lid,
+// ghost layers, pressure boundary), FGH, poisson (right-hand side and solve)
+// and update_uvwp (src/ns_cube.cpp:27-50).
+int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s, hipEvent_t* ev = nullptr) {
     const Grid& g = N->g;
     const dim3 b2(64, 4);
+    auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], s) : hipSuccess; };
     (void)hipGetLastError();
+    NS_HIPCHK(mark(0));
     hipLaunchKernelGGL(ns_lid_kernel, dim3((g.nz + 3 + 63) / 64, g.ny + 2), dim3(64), 0, s, g, N->f.u);
     NS_HIPCHK(hipGetLastError());
     const int fa = std::max(g.nx, g.ny) + 2, fb = std::max(g.nz, g.ny) + 2;
@@ -257,8 +307,10 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s) {
     hipLaunchKernelGGL(ns_pface_kernel, dim3((pa + 63) / 64, pb, 3), dim3(64), 0, s, g, N->f.u, N->f.v, N->f.w,
                        N->f.p);
     NS_HIPCHK(hipGetLastError());
+    NS_HIPCHK(mark(1));
     hipLaunchKernelGGL(ns_fgh_kernel, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1), b2, 0, s, g, N->f);
     NS_HIPCHK(hipGetLastError());
+    NS_HIPCHK(mark(2));
     hipLaunchKernelGGL(ns_rhs_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
     NS_HIPCHK(hipGetLastError());
     // FP32+refine: from the second step on, x still holds the previous step's
@@ -278,9 +330,20 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s) {
                            N->plan->rtol, N->d_bad);
         NS_HIPCHK(hipGetLastError());
     }
+    NS_HIPCHK(mark(3));
     hipLaunchKernelGGL(ns_update_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
     NS_HIPCHK(hipGetLastError());
+    NS_HIPCHK(mark(4));
     return FDMGPU_OK;
+}
+
+// fdmgpu_get_stats on the solver plan reports the last step's pressure solve
+void ns_note_steps(fdmgpu_ns_t* N, hipStream_t s) {
+    if (N->steps == 0) return;
+    N->plan->last_stream = s;
+    N->plan->last_count = 1;
+    N->plan->last_iters = N->plan->w.iters;
+    N->plan->last_resid = N->plan->w.resid;
 }
 
 void ns_free(fdmgpu_ns_t* N) {
@@ -292,6 +355,9 @@ void ns_free(fdmgpu_ns_t* N) {
     double* a[] = {N->f.u, N->f.v, N->f.w, N->f.p, N->f.x, N->f.F, N->f.G, N->f.H, N->f.RHS};
     for (double* p : a) (void)hipFree(p);
     (void)hipFree(N->d_bad);
+    (void)hipFree(N->d_max);
+    for (hipEvent_t e : N->ev_phase)
+        if (e) (void)hipEventDestroy(e);
     if (N->plan) fdmgpu_destroy(N->plan);
     if (N->ev_in) (void)hipEventDestroy(N->ev_in);
     if (N->ev_out) (void)hipEventDestroy(N->ev_out);
@@ -386,14 +452,56 @@ int fdmgpu_ns_step(fdmgpu_ns N, int count, void* stream) {
         N->steps++;
         N->warm_ok = true;
     }
-    if (N->steps > 0) {   // fdmgpu_get_stats on the solver plan reports the last step's pressure solve
-        N->plan->last_stream = s;
-        N->plan->last_count = 1;
-        N->plan->last_iters = N->plan->w.iters;
-        N->plan->last_resid = N->plan->w.resid;
-    }
+    ns_note_steps(N, s);
     NS_HIPCHK(hipEventRecord(N->ev_out, s));
     NS_HIPCHK(hipStreamWaitEvent(user, N->ev_out, 0));
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_step_timed(fdmgpu_ns N, double* ms, void* stream) {
+    if (!N || !ms) return FDMGPU_ERR_INVALID_ARG;
+    NS_HIPCHK(hipSetDevice(N->device));
+    for (hipEvent_t& e : N->ev_phase)
+        if (!e) NS_HIPCHK(hipEventCreate(&e));
+    hipStream_t user = (hipStream_t)stream;
+    hipStream_t s = N->cap;
+    NS_HIPCHK(hipEventRecord(N->ev_in, user));
+    NS_HIPCHK(hipStreamWaitEvent(s, N->ev_in, 0));
+    // eager: the same kernels with the same arguments as a replayed step
+    int rc = ns_enqueue_step(N, s, N->ev_phase);
+    if (rc) return rc;
+    N->steps++;
+    N->warm_ok = true;
+    ns_note_steps(N, s);
+    NS_HIPCHK(hipEventRecord(N->ev_out, s));
+    NS_HIPCHK(hipStreamWaitEvent(user, N->ev_out, 0));
+    NS_HIPCHK(hipEventSynchronize(N->ev_phase[4]));
+    for (int i = 0; i < 4; i++) {
+        float t = 0.f;
+        NS_HIPCHK(hipEventElapsedTime(&t, N->ev_phase[i], N->ev_phase[i + 1]));
+        ms[i] = t;
+    }
+    return FDMGPU_OK;
+}
+
+int fdmgpu_ns_maxabs(fdmgpu_ns N, double* out) {
+    if (!N || !out) return FDMGPU_ERR_INVALID_ARG;
+    NS_HIPCHK(hipSetDevice(N->device));
+    if (!N->d_max) NS_HIPCHK(hipMalloc(&N->d_max, sizeof(double) * 9 * (kMaxabsBlocks + 1)));
+    FieldSet fs{};
+    for (int i = 0; i < 9; i++) {
+        fs.a[i] = *ns_field(N, i);
+        fs.n[i] = N->count[i];
+    }
+    hipStream_t s = N->cap;
+    double* res = N->d_max + 9 * kMaxabsBlocks;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(ns_maxabs_part_kernel, dim3(kMaxabsBlocks, 9), dim3(kMaxabsThreads), 0, s, fs, N->d_max);
+    NS_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(ns_maxabs_final_kernel, dim3(1), dim3(64), 0, s, (const double*)N->d_max, res);
+    NS_HIPCHK(hipGetLastError());
+    NS_HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
+    NS_HIPCHK(hipStreamSynchronize(s));
     return FDMGPU_OK;
 }
 

@@ -336,6 +336,19 @@ int fdmgpu_ns_get(fdmgpu_ns ns, int which, double* host, unsigned long long coun
 int fdmgpu_ns_set(fdmgpu_ns ns, int which, const double* host, unsigned long long count);
 /* Element count and device address of a field (device-side post-processing). */
 int fdmgpu_ns_info(fdmgpu_ns ns, int which, unsigned long long* count, void** device_ptr);
+/* One step, enqueued eagerly (the kernels and arguments of a replayed step,
+ * so the fields are bitwise those of fdmgpu_ns_step), with GPU events at the
+ * bounds of the reference's four phases; waits for it and writes their
+ * durations in ms to ms[0..3]: init_bound (lid, ghost layers, pressure
+ * boundary), FGH, poisson (right-hand side and the solve), update_uvwp.
+ * Replaces the wall-clock timers of NSCube::step() (src/ns_cube.cpp:27-61)
+ * that feed the "Step times (ms)" line of ~NSCube (src/ns_cube.cpp:15-24). */
+int fdmgpu_ns_step_timed(fdmgpu_ns ns, double* ms, void* stream);
+/* The largest |value| of each field array, ghost layers included, in
+ * FDMGPU_NS_* order (out[0..8]); NaNs are skipped, as tensor::maxabs
+ * (src/tensor.h:269-273) skips them.  Reduced on the device, synchronises.
+ * Feeds the verbose per-step line of NSCube::step() (src/ns_cube.cpp:52-56). */
+int fdmgpu_ns_maxabs(fdmgpu_ns ns, double* out);
 /* The pressure solver plan (statistics, timing, refinement settings). */
 int fdmgpu_ns_solver(fdmgpu_ns ns, fdmgpu_plan* plan);
 int fdmgpu_ns_destroy(fdmgpu_ns ns);

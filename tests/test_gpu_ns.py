@@ -115,3 +115,36 @@ def test_ns_errors(fdm):
     with pytest.raises(fdm.FdmGpuError) as e:      # Poisson transform length 13
         fdm.NSCube(12, 12, 12, solver="f64")
     assert e.value.code == fdm.ERR_INVALID_SIZE
+
+
+@pytest.mark.parametrize("solver", ["f64", "refine"])
+def test_ns_timed_steps_and_maxabs(fdm, solver):
+    """fdmgpu_ns_step_timed: the same fields, bitwise, as fdmgpu_ns_step (eager
+    first steps and graph replays alike), four positive phase times;
+    fdmgpu_ns_maxabs: max |value| per whole array, NaN skipped
+    (tensor::maxabs, reference src/tensor.h:269-273)."""
+    n = 31
+    say("ns timed steps + maxabs", solver)
+    kw = dict(u0=1.0, Re=250.0, dt=0.01, solver=solver)
+    a = fdm.NSCube(n, n, 15, **kw)
+    b = fdm.NSCube(n, n, 15, **kw)
+    for _ in range(5):
+        ms = a.step_timed()
+        assert len(ms) == 4 and all(t > 0 for t in ms), ms
+    b.step(5)
+    assert a.time_index == b.time_index == 5
+    m = a.maxabs()
+    for name in fdm.NS_FIELDS:
+        fa = a.get(name)
+        assert np.array_equal(fa, b.get(name)), name
+        assert m[name] == float(np.abs(fa).max()), name
+    # NaN and -inf planted in a ghost face: maxabs skips the NaN, takes |-inf|
+    u = a.get("u")
+    u[0, 0, 0] = np.nan
+    a.set("u", u)
+    assert a.maxabs()["u"] == float(np.nanmax(np.abs(u)))
+    u[0, 0, 1] = -np.inf
+    a.set("u", u)
+    assert a.maxabs()["u"] == np.inf
+    a.close()
+    b.close()
