@@ -182,119 +182,360 @@ def algorithmic_bytes(kernel: str, mode: str, iters: int, n: int, batch: int, fu
     return (first + second + later * max(0, k - 2) + {"residual": 16}.get(kernel, 0)) * pts
 
 
-# Kernels of the 127^3 batch-32 workload as rocprofv3 names them: (key, label,
-# symbol prefix, algorithmic bytes per point per launch).  Both FP32->FP32
-# plane launches (solve 1's inverse into the FP32 iterate, solve 2's forward
-# of the residual) are one symbol with the same bytes.
-PROFILE_KERNELS = [
-    ("xy_fwd", "forward planes, f64 -> f32 (solve 1)", "plane_dst_kernel<double, float, float, 7, 0, 0>", 12),
-    ("plane_f32", "planes f32 -> f32 (solve 1 inverse, solve 2 forward)", "plane_dst_kernel<float, float, float, 7, 0, 0>", 8),
-    ("zcr", "z cyclic reduction (FP32)", "zcr_strip_kernel<float, float, 7, 1, 16>", 8),
-    ("residual_rhs", "FP64 residual -> FP32 rhs (solve 2)", "residual_tile_kernel<128, 8, true, float>", 16),
-    ("xy_inv_acc", "inverse planes, u = u1 + c (solve 2)", "plane_dst_kernel<float, double, float, 7, 2, 0>", 16),
-    ("residual", "FP64 residual, norm only", "residual_tile_kernel<128, 8, false, double>", 16),
-    ("xy_f64", "planes FP64 (plain solve)", "plane_dst_kernel<double, double, double, 7, 0, 0>", 16),
-    ("zcr_f64", "z cyclic reduction (FP64)", "zcr_strip_kernel<double, double, 7, 1, 16>", 16),
-]
-# timing class -> profile key, where they differ
-PROFILE_KEY_OF_CLASS = {"xy_inv": "plane_f32", "xy_fwd_r32": "plane_f32"}
+# ---------------------------------------------------------------------------
+# rocprofv3 summaries (profiles/): no GPU needed, reads the csv output of the
+# commands in PROF_COMMANDS, run from the repository root on one MI355X
+# ---------------------------------------------------------------------------
+PROF_SQ = "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_WAVE_CYCLES SQ_WAIT_INST_ANY"
+# tag -> profiled program (after `--`); kt_* are kernel traces, fetch_/write_/sq_ counter passes
+PROF_COMMANDS = {
+    "kt_head": "python3 bench.py --steps 5 --warmup 1 --no-cpu --legs head",
+    "fetch_head_refine": "python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
+    "write_head_refine": "python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
+    "sq_head_refine": "python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
+    "fetch_head_f64": "python3 bench.py --mode f64 --steps 2 --warmup 1 --no-cpu",
+    "write_head_f64": "python3 bench.py --mode f64 --steps 2 --warmup 1 --no-cpu",
+    "sq_head_f64": "python3 bench.py --mode f64 --steps 2 --warmup 1 --no-cpu",
+    "kt_c2": "python3 bench.py --only c2",
+    "fetch_c2": "python3 bench.py --only c2",
+    "write_c2": "python3 bench.py --only c2",
+    "sq_c2": "python3 bench.py --only c2",
+    "kt_c5": "python3 bench.py --only c5 --steps 40",
+    "fetch_c5": "python3 bench.py --only c5 --steps 40",
+    "write_c5": "python3 bench.py --only c5 --steps 40",
+    "fetch_copy": "python3 bench.py --calib-copy-mib 256",
+    "write_copy": "python3 bench.py --calib-copy-mib 256",
+    "kt_mfma": "fdm_amd/probes/dft16_mfma_probe",
+}
 
 
-def profile_summary(kt: str, pmc: str, n: int = 127, batch: int = 32):
-    """Writes profiles/lapl_cube_127_batch32_kernels.md and
-    profiles/hbm_traffic_lapl_cube_127_batch32.json from the rocprofv3 csv
-    output of the commands listed in the markdown (kernel trace in `kt`,
-    FETCH_SIZE / WRITE_SIZE passes and their copy calibration in `pmc`)."""
+def prof_command(tag: str, d: str) -> str:
+    kind = tag.split("_")[0]
+    if kind == "kt":
+        opts = "--kernel-trace --stats"
+    else:
+        opts = "--pmc " + {"fetch": "FETCH_SIZE", "write": "WRITE_SIZE", "sq": PROF_SQ}[kind]
+    return f"rocprofv3 {opts} --output-format csv -d {d}/{tag} -o {tag} -- {PROF_COMMANDS[tag]}"
+
+
+def kernel_symbol(name: str) -> str:
+    """rocprofv3's kernel name without return type, namespaces and arguments."""
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("fdmgpu::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0:
+            return name[:i].strip()
+    return name.strip()
+
+
+def _targs(sym: str):
+    if "<" not in sym:
+        return sym, []
+    base, rest = sym.split("<", 1)
+    rest = rest[:rest.rfind(">")]
+    args, depth, cur = [], 0, ""
+    for ch in rest:
+        if ch == "," and depth == 0:
+            args.append(cur.strip())
+            cur = ""
+            continue
+        depth += ch == "<"
+        depth -= ch == ">"
+        cur += ch
+    args.append(cur.strip())
+    return base, args
+
+
+def bytes_per_point(sym: str):
+    """Algorithmic HBM bytes per solution point of one full launch of `sym`:
+    every input array read once and every output written once (the stencil
+    and transform neighbours of a point come from LDS or cache, never twice
+    from HBM).  None for kernels without a per-point model (O(1) or surface)."""
+    sz = {"double": 8, "float": 4}
+    base, a = _targs(sym)
+    if base in ("plane_dst_kernel", "plane_rect_kernel"):
+        acc = int(a[4] if base == "plane_dst_kernel" else a[5])
+        return sz[a[0]] + sz[a[1]] + {0: 0, 1: sz[a[1]], 2: 4}[acc]
+    if base == "plane_stream_kernel":
+        return 16
+    if base == "plane_resid_dst_kernel":       # f, u in; FP32 transformed residual out
+        return 8 + sz[a[0]] + 4
+    if base == "xaxis_dst_kernel":
+        return sz[a[0]] + sz[a[1]] + (sz[a[1]] if a[4] == "true" else 0)
+    if base == "yaxis_dst_kernel":
+        return sz[a[0]] + sz[a[1]]
+    if base in ("zcr_strip_kernel", "zcr_kernel", "zcrp_kernel"):
+        return 2 * sz[a[0]]
+    if base == "residual_tile_kernel":         # f, u in; FP32 residual out when WRITE
+        return 8 + sz[a[3]] + (4 if a[2] == "true" else 0)
+    # NSCube step: F,G,H from u,v,w; RHS from F,G,H; u,v,w,p from F,G,H and x
+    return {"ns_fgh_kernel": 48, "ns_rhs_kernel": 32, "ns_update_kernel": 64}.get(base)
+
+
+def _read_csv(path):
     import csv
+    with open(path) as fh:
+        return list(csv.DictReader(fh))
+
+
+def _trace(d, tag):
+    """kernel trace -> (list of (symbol, start_ns, dur_ms), stats rows)."""
+    rows = _read_csv(os.path.join(d, tag, f"{tag}_kernel_trace.csv"))
+    tr = [(kernel_symbol(r["Kernel_Name"]), int(r["Start_Timestamp"]),
+           (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6) for r in rows]
+    tr.sort(key=lambda t: t[1])
+    return tr, _read_csv(os.path.join(d, tag, f"{tag}_kernel_stats.csv"))
+
+
+def _counters(d, tags):
+    """counter passes -> {symbol: [ {counter: value} per dispatch ]} (+ resources)."""
     import collections
+    per = {}
+    res = {}
+    for tag in tags:
+        path = os.path.join(d, tag, f"{tag}_counter_collection.csv")
+        if not os.path.exists(path):
+            continue
+        for r in _read_csv(path):
+            key = (tag, r["Dispatch_Id"])
+            sym = kernel_symbol(r["Kernel_Name"])
+            per.setdefault(key, (sym, {}))[1][r["Counter_Name"]] = float(r["Counter_Value"])
+            res[sym] = {"vgpr": int(r["VGPR_Count"]), "agpr": int(r.get("Accum_VGPR_Count") or 0),
+                        "sgpr": int(r["SGPR_Count"]), "lds_bytes": int(r["LDS_Block_Size"]),
+                        "wg": int(r["Workgroup_Size"]), "grid": int(r["Grid_Size"])}
+    out = collections.defaultdict(list)
+    for sym, c in per.values():
+        out[sym].append(c)
+    return out, res
 
-    pts = float(n) ** 3 * batch
 
-    def key_of(name):
-        for key, _, sym, _ in PROFILE_KERNELS:
-            if sym in name:
-                return key
-        return None
+def _full(values, frac):
+    """the dispatches that did the work: early-exit launches (problems already
+    converged read one flag) are far below the symbol's largest."""
+    if not values:
+        return []
+    top = max(values)
+    return [v for v in values if v >= frac * top]
 
-    trace = list(csv.DictReader(open(os.path.join(kt, "kt_kernel_trace.csv"))))
-    stats = list(csv.DictReader(open(os.path.join(kt, "kt_kernel_stats.csv"))))
+
+def _kernel_table(tr, cnt, pts, kf, kw, t0=None, t1=None):
+    """rows per symbol of the trace window [t0, t1): durations, model bytes, traffic."""
+    import collections
     dur = collections.defaultdict(list)
-    for r in trace:
-        k = key_of(r["Kernel_Name"])
-        if k:
-            dur[k].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
-
-    def counter(name, pre=""):
-        path = os.path.join(pmc, pre + name, f"{pre}{name}_counter_collection.csv")
-        agg = collections.defaultdict(list)
-        for r in csv.DictReader(open(path)):
-            k = "copy" if "copyBuffer" in r["Kernel_Name"] else key_of(r["Kernel_Name"])
-            if k:
-                agg[k].append(float(r["Counter_Value"]))
-        return agg
-
-    fetch, write = counter("FETCH_SIZE"), counter("WRITE_SIZE")
-    cf, cw = counter("FETCH_SIZE", "copy_"), counter("WRITE_SIZE", "copy_")
-    copy_kib = 256 * 1024
-    kf = copy_kib / statistics.mean(cf["copy"])
-    kw = copy_kib / statistics.mean(cw["copy"])
-    out = {"n": n, "batch": batch, "fetch_scale": kf, "write_scale": kw, "kernels": {}}
-    rows = []
-    for key, label, sym, bpp in PROFILE_KERNELS:
-        d = [x for x in dur.get(key, []) if x > 0.02]   # early-exit launches (converged problems) left out
-        fl, wl = fetch.get(key, []), write.get(key, [])
-        f = [x for x in fl if fl and x > 0.05 * max(fl)]
-        w = [x for x in wl if wl and x > 0.05 * max(wl)]
-        alg = bpp * pts
-        rec = {"symbol": sym, "alg_bytes_per_launch": alg}
-        if d:
-            rec.update(avg_ms=statistics.mean(d), launches=len(d))
+    for sym, ts, ms in tr:
+        if (t0 is None or ts >= t0) and (t1 is None or ts < t1):
+            dur[sym].append(ms)
+    rows = {}
+    for sym, ds in dur.items():
+        bpp = bytes_per_point(sym)
+        full = _full(ds, 0.25)
+        if bpp is not None:
+            # a launch that did its work cannot beat the HBM peak by more than the
+            # caches allow: faster ones only read the converged flags
+            full = [x for x in full if x >= bpp * pts / (1.5 * HBM_PEAK_GBS * 1e9) * 1e3]
+        rec = {"dispatches": len(ds), "full": len(full), "total_ms": sum(ds),
+               "avg_ms": statistics.mean(full) if full else statistics.mean(ds)}
+        if bpp is not None and full:
+            rec["bytes_per_point"] = bpp
+            rec["alg_bytes_per_launch"] = bpp * pts
+            rec["achieved_GBs"] = bpp * pts / (rec["avg_ms"] / 1e3) / 1e9
+            rec["frac"] = rec["achieved_GBs"] / HBM_PEAK_GBS
+        f = _full([c["FETCH_SIZE"] for c in cnt.get(sym, []) if "FETCH_SIZE" in c], 0.1)
+        w = _full([c["WRITE_SIZE"] for c in cnt.get(sym, []) if "WRITE_SIZE" in c], 0.1)
         if f and w:
-            rec.update(fetch_kib=statistics.mean(f), write_kib=statistics.mean(w),
-                       traffic_bytes_per_launch=round((statistics.mean(f) * kf + statistics.mean(w) * kw) * 1024))
-        out["kernels"][key] = rec
-        if d:
-            gbs = alg / (rec["avg_ms"] / 1e3) / 1e9
-            tr = rec.get("traffic_bytes_per_launch")
-            rows.append(f"| {label} | `{sym}` | {len(d)} | {rec['avg_ms']:.4f} | {alg / 1e6:.1f} | {gbs:.0f} | "
-                        f"{gbs / HBM_PEAK_GBS:.3f} | " + (f"{tr / 1e6:.1f} ({tr / alg:.2f}x)" if tr else "-") + " |")
+            rec["fetch_kib"] = statistics.mean(f)
+            rec["write_kib"] = statistics.mean(w)
+            rec["traffic_bytes_per_launch"] = round((rec["fetch_kib"] * kf + rec["write_kib"] * kw) * 1024)
+        rows[sym] = rec
+    return rows
+
+
+def _kernel_md(rows, label=None):
+    md = ["| kernel | dispatches (full) | avg ms (full) | bytes / point | algorithmic MB / launch | achieved GB/s | "
+          "of 8 TB/s | HBM traffic MB / launch (x alg) |", "|---|---|---|---|---|---|---|---|"]
+    for sym, r in sorted(rows.items(), key=lambda kv: -kv[1]["total_ms"]):
+        if r["avg_ms"] < 0.002 and "bytes_per_point" not in r:
+            continue
+        if bytes_per_point(sym) is not None and not r["full"]:
+            md.append(f"| `{sym}` | {r['dispatches']} (0) | {r['avg_ms']:.4f} | early exits only (converged "
+                      "problems) | - | - | - | - |")
+            continue
+        alg = r.get("alg_bytes_per_launch")
+        tr = r.get("traffic_bytes_per_launch")
+        md.append(f"| `{sym}` | {r['dispatches']} ({r['full']}) | {r['avg_ms']:.4f} | "
+                  + (f"{r['bytes_per_point']} | {alg / 1e6:.1f} | {r['achieved_GBs']:.0f} | {r['frac']:.3f} | "
+                     if alg else "- | - | - | - | ")
+                  + (f"{tr / 1e6:.1f}" + (f" ({tr / alg:.2f}x)" if alg else "") if tr else "-") + " |")
+    return md
+
+
+def _sq_md(cnt, res, pts):
+    md = ["| kernel | VGPR (+AGPR) | static LDS KiB / block | waves / launch | VALU / 1k pts | SALU / 1k pts | "
+          "LDS insts / 1k pts | VALU / wave | LDS insts / wave | bank-conflict cycles / LDS inst | "
+          "wait share |", "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for sym, disp in sorted(cnt.items()):
+        full = [c for c in disp if "SQ_INSTS_VALU" in c]
+        top = max((c["SQ_INSTS_VALU"] for c in full), default=0)
+        full = [c for c in full if c["SQ_INSTS_VALU"] >= 0.1 * top]
+        if not full or top == 0 or bytes_per_point(sym) is None:
+            continue
+        m = {k: statistics.mean(c[k] for c in full) for k in full[0]}
+        k = 1000.0 / pts
+        r = res[sym]
+        lds = m["SQ_INSTS_LDS"]
+        md.append(f"| `{sym}` | {r['vgpr']} (+{r['agpr']}) | {r['lds_bytes'] / 1024:.1f} | {m['SQ_WAVES']:.0f} | "
+                  f"{m['SQ_INSTS_VALU'] * k:.1f} | {m['SQ_INSTS_SALU'] * k:.1f} | {lds * k:.1f} | "
+                  f"{m['SQ_INSTS_VALU'] / m['SQ_WAVES']:.0f} | {lds / m['SQ_WAVES']:.0f} | "
+                  + (f"{m['SQ_LDS_BANK_CONFLICT'] / lds:.2f}" if lds else "-") + " | "
+                  f"{m['SQ_WAIT_INST_ANY'] / m['SQ_WAVE_CYCLES']:.2f} |")
+    return md
+
+
+def profile_summary(d: str):
+    """Writes the rocprofv3 summaries under profiles/ from the csv output that
+    the commands of PROF_COMMANDS leave in `d` (the kernel trace of each
+    workload, FETCH_SIZE / WRITE_SIZE passes with their copy calibration, SQ
+    instruction counters).  Needs no GPU."""
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
-    with open(os.path.join(ROOT, "profiles", "hbm_traffic_lapl_cube_127_batch32.json"), "w") as fh:
-        json.dump(out, fh, indent=1)
-    md = [
-        "# LaplCube 127^3, batch 32: kernel trace and HBM traffic (MI355X)\n",
-        "Workload: 32 independent Dirichlet 127^3 problems per step (N = 128), seeded manufactured fields, "
-        "FP32+refine to 1e-10 (2 inner solves, default kernel variants) and plain FP64.\n",
-        "Commands (from the repository root, one MI355X; the last one writes this file and the json beside it):\n",
-        "```",
-        f"rocprofv3 --kernel-trace --stats --output-format csv -d {kt} -o kt -- python3 bench.py --steps 5 --warmup 1 --no-cpu --legs head",
-        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {pmc}/FETCH_SIZE -o FETCH_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
-        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {pmc}/WRITE_SIZE -o WRITE_SIZE -- python3 bench.py --mode refine --steps 2 --warmup 1 --no-cpu",
-        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {pmc}/copy_FETCH_SIZE -o copy_FETCH_SIZE -- python3 bench.py --calib-copy-mib 256",
-        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {pmc}/copy_WRITE_SIZE -o copy_WRITE_SIZE -- python3 bench.py --calib-copy-mib 256",
-        f"python3 bench.py --profile-summary {kt} {pmc}",
-        "```\n",
-        f"Calibration: a 256 MiB device-to-device copy reads FETCH_SIZE = {copy_kib / kf:.0f} KiB and WRITE_SIZE = "
-        f"{copy_kib / kw:.0f} KiB, so FETCH_SIZE is scaled by {kf:.2f} and WRITE_SIZE by {kw:.2f} to get bytes.\n",
-        "Durations are per dispatch from the kernel trace.  Dispatches shorter than 20 us (launches for problems "
-        "already converged, which read one flag and exit) are left out, and so are their counter readings.\n",
-        "| kernel | symbol | dispatches | avg ms | algorithmic MB / launch | achieved GB/s | of 8 TB/s | "
-        "HBM traffic MB / launch (x alg) |",
-        "|---|---|---|---|---|---|---|---|",
-    ] + rows + ["", "rocprofv3 --stats summary (all dispatches, warm-up included):", "",
-                "| kernel | calls | avg ns | % |", "|---|---|---|---|"]
-    for r in stats:
-        md.append(f"| `{r['Name'].split('(')[0].replace('void ', '')}` | {r['Calls']} | {float(r['AverageNs']):.0f} | "
-                  f"{float(r['Percentage']):.2f} |")
-    with open(os.path.join(ROOT, "profiles", "lapl_cube_127_batch32_kernels.md"), "w") as fh:
-        fh.write("\n".join(md) + "\n")
-    print("\n".join(md))
+    have = lambda tag: os.path.isdir(os.path.join(d, tag))
+
+    def cmds(tags):
+        return ["```"] + [prof_command(t, d) for t in tags if have(t)] + [f"python3 bench.py --profile-summary {d}", "```", ""]
+
+    copy_kib = 256 * 1024
+    cc, _ = _counters(d, ["fetch_copy", "write_copy"])
+    copy = cc.get("__amd_rocclr_copyBuffer", [])
+    kf = copy_kib / statistics.mean(c["FETCH_SIZE"] for c in copy if "FETCH_SIZE" in c)
+    kw = copy_kib / statistics.mean(c["WRITE_SIZE"] for c in copy if "WRITE_SIZE" in c)
+    calib = (f"Traffic calibration: a 256 MiB device-to-device copy reads FETCH_SIZE = {copy_kib / kf:.0f} KiB and "
+             f"WRITE_SIZE = {copy_kib / kw:.0f} KiB per dispatch, so FETCH_SIZE x {kf:.2f} and WRITE_SIZE x {kw:.2f} "
+             "(KiB -> bytes x 1024) give HBM bytes.")
+    rules = ("Algorithmic bytes per point: each launch reads its inputs and writes its outputs once (element sizes "
+             "from the template arguments: FP64 8, FP32 4; accumulating launches also read the output; the refinement's "
+             "first-iterate inverse reads an FP32 base; the residual reads f and u and writes the FP32 rhs when its "
+             "WRITE flag is set).  Averages are over full dispatches only: launches for problems already converged "
+             "read one flag and exit, and dispatches under a quarter of the symbol's longest are counted apart.")
+    written = []
+
+    # ---- 127^3 x 32: the headline workload, FP32+refine and FP64
+    n, batch = 127, 32
+    pts = float(n) ** 3 * batch
+    if have("kt_head"):
+        tr, stats = _trace(d, "kt_head")
+        cnt, res = _counters(d, ["fetch_head_refine", "write_head_refine", "fetch_head_f64", "write_head_f64"])
+        rows = _kernel_table(tr, cnt, pts, kf, kw)
+        classes = {"xy_fwd": "plane_dst_kernel<double, float, float, 7, 0, 0>",
+                   "xy_inv": "plane_dst_kernel<float, float, float, 7, 0, 0>",
+                   "xy_fwd_r32": "plane_dst_kernel<float, float, float, 7, 0, 0>",
+                   "zcr": "zcr_strip_kernel<float, float, 7, 1, 16>",
+                   "residual_rhs": "residual_tile_kernel<128, 8, true, float>",
+                   "xy_inv_acc": "plane_dst_kernel<float, double, float, 7, 2, 0>",
+                   "residual": "residual_tile_kernel<128, 8, false, double>"}
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_lapl_cube_127_batch32.json"), "w") as fh:
+            json.dump({"n": n, "batch": batch, "fetch_scale": kf, "write_scale": kw,
+                       "classes": {c: s for c, s in classes.items() if s in rows}, "kernels": rows}, fh, indent=1)
+        md = ["# LaplCube 127^3, batch 32: kernel trace and HBM traffic (MI355X)", "",
+              "Workload: 32 independent Dirichlet 127^3 problems per step (N = 128), seeded manufactured fields; "
+              "FP32+refine to 1e-10 (2 inner solves, default kernel variants) and plain FP64 (streaming plane kernel).",
+              "", "Commands (repository root, one MI355X; the last writes this file and the json beside it):", ""]
+        md += cmds(["kt_head", "fetch_head_refine", "write_head_refine", "fetch_head_f64", "write_head_f64",
+                    "fetch_copy", "write_copy"])
+        md += [calib, "", rules, ""] + _kernel_md(rows)
+        md += ["", "rocprofv3 --stats (all dispatches, warm-up included):", "", "| kernel | calls | avg ns | % |",
+               "|---|---|---|---|"]
+        md += [f"| `{kernel_symbol(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']):.0f} | "
+               f"{float(r['Percentage']):.2f} |" for r in stats]
+        written.append(("lapl_cube_127_batch32_kernels.md", md))
+    if have("sq_head_refine") or have("sq_head_f64"):
+        cnt, res = _counters(d, ["sq_head_refine", "sq_head_f64"])
+        md = ["# LaplCube 127^3, batch 32: SQ instruction counters per kernel (MI355X)", "",
+              "Per full dispatch (early-exit launches left out).  VALU / SALU / LDS are wave-level instruction counts "
+              "(SQ_INSTS_*); per 1k points = per 1000 solution points of the launch.  Bank-conflict cycles / LDS inst = "
+              "SQ_LDS_BANK_CONFLICT / SQ_INSTS_LDS.  Wait share = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES: the part of the "
+              "waves' resident time spent waiting on any dependency (memory, LDS, or an issue slot).  VGPR, AGPR and "
+              "static LDS as the dispatch records them (the plane kernels' LDS is dynamic: FP64 128^2 planes take "
+              "139 KiB, FP32 ones 69.6 KiB).", "", "Commands:", ""]
+        md += cmds(["sq_head_refine", "sq_head_f64"]) + _sq_md(cnt, res, pts)
+        written.append(("lapl_cube_127_batch32_sq_counters.md", md))
+
+    # ---- 255^3, one problem: axis-tile kernels
+    if have("kt_c2"):
+        n = 255
+        pts = float(n) ** 3
+        tr, stats = _trace(d, "kt_c2")
+        cnt, _ = _counters(d, ["fetch_c2", "write_c2"])
+        rows = _kernel_table(tr, cnt, pts, kf, kw)
+        md = ["# LaplCube 255^3, one problem: kernel trace, HBM traffic, SQ counters (MI355X)", "",
+              "Workload: one Dirichlet 255^3 problem (N = 256, seed 2 manufactured field), plain FP64, then "
+              "FP32+refine to 1e-10 (2 inner solves); a 256^2 plane does not fit LDS, so x and y run as axis-tile "
+              "kernels (x tiles of 32 pencils, y tiles of 32 (FP64) / 64 (FP32) pencils, z CR in 32-row chunks).",
+              "", "Commands:", ""]
+        md += cmds(["kt_c2", "fetch_c2", "write_c2", "sq_c2", "fetch_copy", "write_copy"])
+        md += [calib, "", rules, ""] + _kernel_md(rows)
+        if have("sq_c2"):
+            sq, res = _counters(d, ["sq_c2"])
+            md += ["", "SQ counters (definitions as in lapl_cube_127_batch32_sq_counters.md):", ""] + _sq_md(sq, res, pts)
+        written.append(("lapl_cube_255_single_kernels.md", md))
+
+    # ---- NSCube 127^3 step: six step kernels and the pressure solve
+    if have("kt_c5"):
+        n = 127
+        pts = float(n) ** 3
+        tr, _ = _trace(d, "kt_c5")
+        cnt, _ = _counters(d, ["fetch_c5", "write_c5"])
+        # the FP32+refine cavity runs first; the FP64-solver cavity starts at its
+        # first FP64 plane launch
+        t_split = min((ts for sym, ts, _ in tr if sym.startswith("plane_stream_kernel")
+                       or sym.startswith("plane_dst_kernel<double, double, double")), default=None)
+        md = ["# NSCube 127^3 cavity step: kernel trace, HBM traffic and the step's byte model (MI355X)", "",
+              "Workload: lid-driven cavity 127^3 on [-pi,pi]^3, Re 250, dt 0.01, u0 1, from rest; 40 steps with the "
+              "FP32+refine pressure solve (warm-started from the previous pressure), then 40 with the FP64 solve.  "
+              "Steps 3.. are hipGraph replays.", "", "Commands:", ""]
+        md += cmds(["kt_c5", "fetch_c5", "write_c5", "fetch_copy", "write_copy"])
+        md += [calib, "", rules + "  NSCube kernels per interior point: FGH reads u, v, w and writes F, G, H "
+               "(48 B); RHS reads F, G, H and writes RHS (32 B); the update reads F, G, H, x and writes u, v, w, p "
+               "(64 B); lid, ghost and pressure-face kernels touch O(n^2) points and carry no per-point model.", ""]
+        summary = {}
+        for label, t0, t1 in (("FP32+refine pressure solve", None, t_split), ("FP64 pressure solve", t_split, None)):
+            rows = _kernel_table(tr, cnt, pts, kf, kw, t0, t1)
+            steps = rows.get("ns_fgh_kernel", {}).get("dispatches", 0)
+            if not steps:
+                continue
+            ker_ms = sum(r["total_ms"] for r in rows.values()) / steps
+            alg = sum(r.get("bytes_per_point", 0) * pts * r["full"] for r in rows.values()) / steps
+            traffic = sum(r.get("traffic_bytes_per_launch", 0) * r["full"] for r in rows.values()) / steps
+            summary[label] = (steps, ker_ms, alg, traffic)
+            md += [f"## {label}: {steps} steps", ""] + _kernel_md(rows) + [""]
+            md += [f"Per step: {ker_ms:.4f} ms of kernels; algorithmic {alg / 1e6:.1f} MB ({alg / pts:.0f} B / point) "
+                   f"-> {alg / (ker_ms / 1e3) / 1e9:.0f} GB/s = {alg / (ker_ms / 1e3) / 1e9 / HBM_PEAK_GBS:.3f} of "
+                   f"8 TB/s" + (f"; measured HBM traffic {traffic / 1e6:.1f} MB ({traffic / alg:.2f}x)" if traffic else "")
+                   + ".", ""]
+        written.append(("ns_cube_127_step_kernels.md", md))
+
+    # ---- the MFMA measurement probe
+    if have("kt_mfma"):
+        tr, stats = _trace(d, "kt_mfma")
+        md = ["# 16-point sub-DFT probe: VALU vs MFMA (MI355X)", "",
+              "The plane transforms' 16-point sub-DFTs on the same LDS data, once on the VALU (the production "
+              "code path) and once as v_mfma_f32_32x32x2_f32 products (fdm_amd/probes/dft16_mfma_probe.hip).",
+              "", "Command:", ""] + cmds(["kt_mfma"])
+        md += ["| kernel | calls | avg ns | min ns | max ns |", "|---|---|---|---|---|"]
+        md += [f"| `{kernel_symbol(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']):.0f} | "
+               f"{float(r['MinNs']):.0f} | {float(r['MaxNs']):.0f} |" for r in stats]
+        written.append(("dft16_mfma_probe_kernels.md", md))
+
+    for name, md in written:
+        with open(os.path.join(ROOT, "profiles", name), "w") as fh:
+            fh.write("\n".join(md) + "\n")
+        print(f"wrote profiles/{name}")
 
 
 def measured_traffic(kernel: str, n: int, batch: int):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    when it was taken on this workload; None otherwise."""
+    """HBM bytes per launch of timing class `kernel` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), when it was taken on this workload; None otherwise."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_lapl_cube_127_batch32.json")
     try:
         with open(path) as fh:
@@ -303,7 +544,8 @@ def measured_traffic(kernel: str, n: int, batch: int):
         return None
     if t.get("n") != n or t.get("batch") != batch:
         return None
-    k = t.get("kernels", {}).get(PROFILE_KEY_OF_CLASS.get(kernel, kernel))
+    sym = t.get("classes", {}).get(kernel)
+    k = t.get("kernels", {}).get(sym) if sym else None
     return None if k is None else k.get("traffic_bytes_per_launch")
 
 
@@ -810,14 +1052,18 @@ def main():
     ap.add_argument("--legs", default="head,c2,c3,c4,c5,pub,host,cpu",
                     help="comma list of legs: head (the value; always run), c2, c3, c4, c5, pub, host, cpu")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving rate")
-    ap.add_argument("--profile-summary", nargs=2, metavar=("KT_DIR", "PMC_DIR"),
-                    help="summarise rocprofv3 csv output into profiles/ and exit (no GPU)")
+    ap.add_argument("--profile-summary", metavar="DIR",
+                    help="summarise the rocprofv3 csv output of the PROF_COMMANDS runs in DIR into profiles/ "
+                         "and exit (no GPU)")
+    ap.add_argument("--only", choices=("c2", "c5"),
+                    help="run just this leg, shortened for profilers (c2: 2 solves per precision; c5: "
+                         "--steps NSCube steps, no CPU check), print its JSON and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (calibrates the HBM "
                          "FETCH_SIZE/WRITE_SIZE counters against a copy of known size) and exit")
     args = ap.parse_args()
     if args.profile_summary:
-        profile_summary(*args.profile_summary)
+        profile_summary(args.profile_summary)
         return
     if args.calib_copy_mib:
         from fdm_amd import hip
@@ -828,6 +1074,12 @@ def main():
             hip.check(hip.rt().hipMemcpy(b.ptr, a.ptr, nb, hip.D2D), "copy")
         hip.synchronize()
         print(json.dumps({"calib_copy_bytes": nb, "copies": 3}))
+        return
+    if args.only == "c2":
+        print(json.dumps(leg_c2(reps=2)), flush=True)
+        return
+    if args.only == "c5":
+        print(json.dumps(leg_c5(steps=args.steps, check_at=max(2, args.steps // 4), cpu_check=False)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding "
