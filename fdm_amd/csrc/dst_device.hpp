@@ -221,6 +221,8 @@ __device__ __forceinline__ int opaque(int x) {
 template <int P, int N1, bool ALONG_X>
 struct PlaneLayout {
     using Sw = RowSwizzle<N1>;
+    // pencil of job jj (and jj + JOBS): jobs in order
+    __device__ static constexpr int pencil(int jj) { return jj; }
     __device__ static constexpr int at(int row, int col) { return row * P + col + Sw::f(row); }
     __device__ static constexpr int addr(int pencil, int slot) {
         return ALONG_X ? at(pencil, slot) : at(slot, pencil);
@@ -246,13 +248,86 @@ struct PlaneLayout {
     }
 };
 
+// FP64 plane of N = 128 (N1 = 8, 64 jobs per pass): no row swizzle, a pitch
+// P = 2 (mod 32) slots, and jobs dealt to pencils out of order so that both
+// the b64 reads (32 eight-byte banks per half-wavefront: 4 jobs) and the b64
+// writes (16 per quarter-wavefront: 2 jobs) of either pass are
+// conflict-free.  A job's 8 lanes touch slots 8m + (t ^ c) of its pencils
+// (m fixed or t-dependent): along a row those are 8 distinct banks, down a
+// column rows r with 2r mod 32 spread over 8 distinct even banks.  So
+//   x pass (pencils = rows): the 4 jobs of a half-wavefront take rows
+//     e, e+4, e+8, e+12 (2r = 2e + 0, 8, 16, 24 mod 32), the 2 of a
+//     quarter-wavefront rows 4 apart (8 banks apart mod 16);
+//   y pass (pencils = columns): the 4 jobs take columns c, c+1, c+16, c+17
+//     (odd/even and low/high halves of the 32 banks), the 2 of a
+//     quarter-wavefront c, c+1.
+// 128 rows of 130 slots: 133 KB for a FP64 plane (the swizzled layout's
+// pitch 136 takes 139 KB).
+template <int P, bool ALONG_X>
+struct PlaneLayoutB64 {
+    static_assert(P % 32 == 2, "pitch 2 mod 32");
+    __device__ static constexpr int pencil(int jj) {
+        const int q = jj >> 2, i = jj & 3;
+        return ALONG_X ? 16 * (q >> 2) + (q & 3) + 4 * i : 32 * (q >> 3) + 2 * (q & 7) + (i & 1) + 16 * (i >> 1);
+    }
+    __device__ static constexpr int at(int row, int col) { return row * P + col; }
+    __device__ static constexpr int addr(int pencil, int slot) {
+        return ALONG_X ? at(pencil, slot) : at(slot, pencil);
+    }
+    template <int DP>
+    __device__ static int addr_split(int pencil, int lv, int cp) {
+        if constexpr (ALONG_X) return opaque(pencil * P + lv) + (DP * P + cp);
+        else return opaque(lv * P + pencil) + (DP + cp * P);
+    }
+};
+
+// v from lane - D within the lane's row of 16 (DPP row_shr:D, a VALU move:
+// no LDS instruction as a shuffle through ds_bpermute would take); lanes
+// whose source falls outside the row get v back.
+template <int D, typename T>
+__device__ __forceinline__ T row_shr(T v) {
+    static_assert(D >= 1 && D <= 15, "row_shr range");
+    if constexpr (sizeof(T) == 8) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x110 + D, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x110 + D, 0xf, 0xf, false);
+        return __builtin_bit_cast(T, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    } else {
+        const int b = __builtin_bit_cast(int, v);
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(b, b, 0x110 + D, 0xf, 0xf, false));
+    }
+}
+
 // Inclusive scan across `width` consecutive lanes (width a power of two <= 64).
+// Within a row of 16 lanes (width <= 16) the lane - d values come by DPP;
+// a lane takes one only when lane - d is in its own group (t >= d), so the
+// row_shr values from the neighbouring group are never used.
 template <int WIDTH, typename T>
 __device__ __forceinline__ T lane_scan_incl(T v) {
+    if constexpr (WIDTH <= 16) {
+        const int t = threadIdx.x & (WIDTH - 1);
+        if constexpr (WIDTH > 1) {
+            const T o = row_shr<1>(v);
+            if (t >= 1) v += o;
+        }
+        if constexpr (WIDTH > 2) {
+            const T o = row_shr<2>(v);
+            if (t >= 2) v += o;
+        }
+        if constexpr (WIDTH > 4) {
+            const T o = row_shr<4>(v);
+            if (t >= 4) v += o;
+        }
+        if constexpr (WIDTH > 8) {
+            const T o = row_shr<8>(v);
+            if (t >= 8) v += o;
+        }
+    } else {
 #pragma unroll
-    for (int d = 1; d < WIDTH; d <<= 1) {
-        T o = __shfl_up(v, d, WIDTH);
-        if ((threadIdx.x & (WIDTH - 1)) >= d) v += o;
+        for (int d = 1; d < WIDTH; d <<= 1) {
+            T o = __shfl_up(v, d, WIDTH);
+            if ((threadIdx.x & (WIDTH - 1)) >= d) v += o;
+        }
     }
     return v;
 }
@@ -313,7 +388,7 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const Lane& L,
     for (int base = 0; base < WORK; base += NT) {
         const int w = base + tid;
         if (w < WORK) {
-            const int g = w / N1;
+            const int g = Lay::pencil(w / N1);
 // slot = lv + cp: lv lane-dependent, cp a compile-time multiple of N1 (see
 // PlaneLayout::addr_split); pencil gb = g + JOBS sits at a constant offset
 #define PA(lv, cp) lds[Lay::template addr_split<0>(g, (lv), (cp))]
@@ -494,7 +569,9 @@ __device__ __forceinline__ void dst_pairs(T* __restrict__ lds, const DstLane<T, 
 // H[c] + (8 slots) * (constant part of m), H[c] = offset of slot t ^ c: one
 // add per distinct (base, c) where the generic path computes the slot and
 // its swizzle per access.
-template <typename T, int P, int JOBS, int NT, bool COLS, typename Lane = DstLane<T, 7>>
+// B64: the plane is a PlaneLayoutB64 (no row swizzle, jobs dealt to pencils
+// by its pencil()) instead of a swizzled PlaneLayout.
+template <typename T, int P, int JOBS, int NT, bool COLS, typename Lane = DstLane<T, 7>, bool B64 = false>
 __device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const Lane& L, const T scale,
                                              const int tid) {
     using S = DstShape<7>;
@@ -507,13 +584,17 @@ __device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const Lane& L,
     constexpr int PBO = COLS ? JOBS * ES : JOBS * P * ES;   // pencil g + JOBS (f(g + JOBS) = f(g))
     static_assert(JOBS % N1 == 0, "swizzle period");
     const int t = tid & (N1 - 1);
-    const int g = tid / N1;
+    const int g = [&] {
+        if constexpr (B64) return PlaneLayoutB64<P, !COLS>::pencil(tid / N1);
+        else return tid / N1;
+    }();
     char* const base = reinterpret_cast<char*>(lds);
-    auto rowb = [](int r) { return COLS ? (r * P + Sw::f(r)) * ES : r * ES; };   // slot r of pencil 0
+    auto fsw = [](int r) { return B64 ? 0 : Sw::f(r); };
+    auto rowb = [&](int r) { return COLS ? (r * P + fsw(r)) * ES : r * ES; };   // slot r of pencil 0
     int H[N1];
 #pragma unroll
     for (int c = 0; c < N1; ++c) H[c] = opaque(rowb(t ^ c));
-    const int pb = COLS ? g * ES : (g * P + Sw::f(g)) * ES;
+    const int pb = COLS ? g * ES : (g * P + fsw(g)) * ES;
     const int b0 = opaque(pb), b1 = opaque(pb + ROW8 * t), b2 = opaque(pb + 2 * ROW8 * t),
               b3 = opaque(pb + ROW8 * (15 - t));
     // the slots outside the 8m + (t ^ c) form: N - t (step 1, j = t) and the

@@ -108,12 +108,45 @@ __global__ void ns_pface_kernel(Grid g, const double* __restrict__ u, const doub
     }
 }
 
+// Tiles of the stencil kernels (64 x 4 threads over x, y; one z-plane each)
+// dealt to the XCDs in contiguous runs.  Workgroups of a 1-D grid go to the 8
+// XCDs round-robin by blockIdx.x; XCD x takes tiles [x per, (x+1) per) in
+// (x tile, y tile, z) order, so a tile's y and z neighbours, which read the
+// same halo rows and planes, run on the same L2 close in time (with the tiles
+// of a plane spread over all XCDs every XCD fetched the halo rows of its
+// tiles and the planes above and below once more).  The grid is padded to a
+// multiple of 8 blocks; blocks past the last tile return.
+constexpr int kXcds = 8;
+constexpr int kTileX = 64, kTileY = 4;
+
+struct TileGrid {
+    int gx, gy, gz;
+    int blocks() const { return (gx * gy * gz + kXcds - 1) / kXcds * kXcds; }
+};
+
+inline TileGrid tile_grid(int nx, int ny, int nz) {
+    return TileGrid{(nx + kTileX - 1) / kTileX, (ny + kTileY - 1) / kTileY, nz};
+}
+
+__device__ __forceinline__ bool xcd_tile(const TileGrid tg, int& bx, int& by, int& bz) {
+    const int per = (int)gridDim.x / kXcds;
+    const int T = ((int)blockIdx.x % kXcds) * per + (int)blockIdx.x / kXcds;
+    if (T >= tg.gx * tg.gy * tg.gz) return false;
+    bx = T % tg.gx;
+    const int r = T / tg.gx;
+    by = r % tg.gy;
+    bz = r / tg.gy;
+    return true;
+}
+
 // F, G, H over the union box i in [0,nz], k in [0,ny], j in [0,nx]
 // (src/ns_cube.cpp:131-201)
-__global__ void ns_fgh_kernel(Grid g, Fields f) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
+__global__ __launch_bounds__(kTileX * kTileY) void ns_fgh_kernel(Grid g, Fields f, const TileGrid tg) {
+    int bx, by, bz;
+    if (!xcd_tile(tg, bx, by, bz)) return;
+    const int j = bx * kTileX + (int)threadIdx.x;
+    const int k = by * kTileY + (int)threadIdx.y;
+    const int i = bz;
     if (j > g.nx || k > g.ny) return;
     const auto U = as_global((const double*)f.u);
     const auto V = as_global((const double*)f.v);
@@ -160,10 +193,12 @@ __global__ void ns_fgh_kernel(Grid g, Fields f) {
 
 // Poisson right-hand side with the pressure boundary values moved over
 // (src/ns_cube.cpp:204-235); interior i, k, j in [1, n]
-__global__ void ns_rhs_kernel(Grid g, Fields f) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 1;
+__global__ __launch_bounds__(kTileX * kTileY) void ns_rhs_kernel(Grid g, Fields f, const TileGrid tg) {
+    int bx, by, bz;
+    if (!xcd_tile(tg, bx, by, bz)) return;
+    const int j = bx * kTileX + (int)threadIdx.x + 1;
+    const int k = by * kTileY + (int)threadIdx.y + 1;
+    const int i = bz + 1;
     if (j > g.nx || k > g.ny) return;
     const auto Fa = as_global((const double*)f.F);
     const auto Ga = as_global((const double*)f.G);
@@ -173,26 +208,25 @@ __global__ void ns_rhs_kernel(Grid g, Fields f) {
     double r = ((Fa[g.F(i, k, j)] - Fa[g.F(i, k, j - 1)]) / g.dx + (Ga[g.G(i, k, j)] - Ga[g.G(i, k - 1, j)]) / g.dy +
                 (Ha[g.H(i, k, j)] - Ha[g.H(i - 1, k, j)]) / g.dz) /
                g.dt;
-    // p has a ghost layer on every side, so the six neighbours of an interior
-    // point are always in the array: load them all (no branch around a load)
-    // and keep the boundary ones
-    const double pzm = P[g.p(i - 1, k, j)], pym = P[g.p(i, k - 1, j)], pxm = P[g.p(i, k, j - 1)];
-    const double pxp = P[g.p(i, k, j + 1)], pyp = P[g.p(i, k + 1, j)], pzp = P[g.p(i + 1, k, j)];
-    if (i <= 1) r -= pzm / dz2;
-    if (k <= 1) r -= pym / dy2;
-    if (j <= 1) r -= pxm / dx2;
-    if (j >= g.nx) r -= pxp / dx2;
-    if (k >= g.ny) r -= pyp / dy2;
-    if (i >= g.nz) r -= pzp / dz2;
+    // the pressure's boundary values, read only at the points next to a face
+    // (p has a ghost layer on every side, so every neighbour is in the array)
+    if (i <= 1) r -= P[g.p(i - 1, k, j)] / dz2;
+    if (k <= 1) r -= P[g.p(i, k - 1, j)] / dy2;
+    if (j <= 1) r -= P[g.p(i, k, j - 1)] / dx2;
+    if (j >= g.nx) r -= P[g.p(i, k, j + 1)] / dx2;
+    if (k >= g.ny) r -= P[g.p(i, k + 1, j)] / dy2;
+    if (i >= g.nz) r -= P[g.p(i + 1, k, j)] / dz2;
     as_global(f.RHS)[g.x(i, k, j)] = r;
 }
 
 // update_uvwp (src/ns_cube.cpp:238-277): interior faces from F, G, H and the
 // pressure x; p's interior = x (p = x copies the index intersection)
-__global__ void ns_update_kernel(Grid g, Fields f) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 1;
+__global__ __launch_bounds__(kTileX * kTileY) void ns_update_kernel(Grid g, Fields f, const TileGrid tg) {
+    int bx, by, bz;
+    if (!xcd_tile(tg, bx, by, bz)) return;
+    const int j = bx * kTileX + (int)threadIdx.x + 1;
+    const int k = by * kTileY + (int)threadIdx.y + 1;
+    const int i = bz + 1;
     if (j > g.nx || k > g.ny) return;
     const auto X = as_global((const double*)f.x);
     // every load is issued before the first use (no branch around a load):
@@ -294,7 +328,7 @@ namespace {
 // and update_uvwp (src/ns_cube.cpp:27-50).
 int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s, hipEvent_t* ev = nullptr) {
     const Grid& g = N->g;
-    const dim3 b2(64, 4);
+    const dim3 b2(kTileX, kTileY);
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], s) : hipSuccess; };
     (void)hipGetLastError();
     NS_HIPCHK(mark(0));
@@ -308,10 +342,11 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s, hipEvent_t* ev = nullptr) {
                        N->f.p);
     NS_HIPCHK(hipGetLastError());
     NS_HIPCHK(mark(1));
-    hipLaunchKernelGGL(ns_fgh_kernel, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1), b2, 0, s, g, N->f);
+    const TileGrid tf = tile_grid(g.nx + 1, g.ny + 1, g.nz + 1), ti = tile_grid(g.nx, g.ny, g.nz);
+    hipLaunchKernelGGL(ns_fgh_kernel, dim3(tf.blocks()), b2, 0, s, g, N->f, tf);
     NS_HIPCHK(hipGetLastError());
     NS_HIPCHK(mark(2));
-    hipLaunchKernelGGL(ns_rhs_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
+    hipLaunchKernelGGL(ns_rhs_kernel, dim3(ti.blocks()), b2, 0, s, g, N->f, ti);
     NS_HIPCHK(hipGetLastError());
     // FP32+refine: from the second step on, x still holds the previous step's
     // pressure, a close iterate for the new right-hand side; the refinement
@@ -331,7 +366,7 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s, hipEvent_t* ev = nullptr) {
         NS_HIPCHK(hipGetLastError());
     }
     NS_HIPCHK(mark(3));
-    hipLaunchKernelGGL(ns_update_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz), b2, 0, s, g, N->f);
+    hipLaunchKernelGGL(ns_update_kernel, dim3(ti.blocks()), b2, 0, s, g, N->f, ti);
     NS_HIPCHK(hipGetLastError());
     NS_HIPCHK(mark(4));
     return FDMGPU_OK;

@@ -182,10 +182,14 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
 // planes q = blockIdx.x, blockIdx.x + gridDim.x, ... (about the same count per
 // block: the grid is sized so), and right after plane q is staged into LDS it
 // issues the global loads of its next plane into registers; they are in
-// flight during the x and y passes and the store phase of plane q.  The
-// transforms, the LDS layout and the order of every floating-point operation
-// are those of plane_dst_kernel, so results are bitwise the same.
+// flight during the x and y passes and the store phase of plane q.  The plane
+// sits in LDS in the b64-conflict-free PlaneLayoutB64 (pitch kStreamPitch,
+// jobs dealt to pencils out of order); each pencil's transform and the order
+// of every floating-point operation in it are those of plane_dst_kernel, so
+// results are bitwise the same.
 // ---------------------------------------------------------------------------
+constexpr int kStreamPitch = 130;
+
 template <int KIND>
 __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu(2))) void plane_stream_kernel(
     const BatchPtr<const double> in, const BatchPtr<double> out, const double sx, const double sy,
@@ -194,8 +198,9 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
     using C = PlaneCfg<LOGN>;
     using S = DstShape<LOGN>;
     using IO = PencilIO<KIND, LOGN>;
-    using LX = PlaneLayout<C::P, S::N1, true>;
-    using LY = PlaneLayout<C::P, S::N1, false>;
+    constexpr int P = kStreamPitch;   // conflict-free b64 layout (PlaneLayoutB64)
+    using LX = PlaneLayoutB64<P, true>;
+    using LY = PlaneLayoutB64<P, false>;
     constexpr int N = C::N, n = IO::n, o = IO::off, NT = C::NT;
     constexpr int R = NT / N, ITER = (n + R - 1) / R;
     static_assert(NT % N == 0, "a block covers whole rows");
@@ -205,14 +210,14 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* lds = reinterpret_cast<double*>(smem_raw);
     constexpr unsigned plane_pts = (unsigned)n * n;
-    const SweepSlots<LOGN, NT, C::P, S::N1, o> slot(tid);
     const int x0 = tid & (N - 1), r0 = tid >> LOGN;
+    const int slot0 = opaque(LX::at(r0 + o, x0 + o));   // element i: slot0 + R i P
     const int lane_in = (int)((r0 * n + x0) * sizeof(double));
 
     // the DST's shifted sines come from an LDS copy behind the plane (the
     // prefetch takes their registers)
     using Lane = DstLaneLdsSine<double, LOGN>;
-    double* sines = lds + N * C::P;
+    double* sines = lds + N * P;
     Lane L;
     Lane::fill_sines(tables, sines, tid, NT);
     L.init(tables, sines, tid & (S::N1 - 1));
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
         if (x0 < n) {
 #pragma unroll
             for (int i = 0; i < ITER; ++i)
-                if (R * i + R <= n || r0 + R * i < n) lds[slot(i)] = v[i];
+                if (R * i + R <= n || r0 + R * i < n) lds[slot0 + R * i * P] = v[i];
         }
         __syncthreads();
         const int qn = q + (int)gridDim.x;
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
         int t2 = tid;
         asm volatile("" : "+v"(t2));
         if constexpr (KIND == kDst)
-            dst_pairs128<double, C::P, N / 2, NT, true, Lane>(lds, L, sy, t2);   // pencils = columns (y)
+            dst_pairs128<double, P, N / 2, NT, true, Lane, true>(lds, L, sy, t2);   // pencils = columns (y)
         else
             pencil_pairs<KIND, double, LOGN, N / 2, LY, NT, Lane>(lds, L, sy, t2);
         __syncthreads();
@@ -412,7 +417,7 @@ template <int KIND>
 static hipError_t plane_stream_k(const PlaneArgs& a) {
     using C = PlaneCfg<7>;
     auto k = plane_stream_kernel<KIND>;
-    const size_t lds = C::template lds_bytes<double>() + sizeof(double) * C::N;   // plane + sine table
+    const size_t lds = sizeof(double) * (size_t)C::N * (kStreamPitch + 1);   // plane + sine table
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     const long total = (long)a.nplanes * a.batch;
