@@ -221,8 +221,10 @@ __device__ __forceinline__ int opaque(int x) {
 template <int P, int N1, bool ALONG_X>
 struct PlaneLayout {
     using Sw = RowSwizzle<N1>;
-    // pencil of job jj (and jj + JOBS): jobs in order
+    // pencil of job jj, and the distance to its second pencil: jobs in
+    // order, pencils jj and jj + JOBS
     __device__ static constexpr int pencil(int jj) { return jj; }
+    static constexpr int pair_delta(int jobs) { return jobs; }
     __device__ static constexpr int at(int row, int col) { return row * P + col + Sw::f(row); }
     __device__ static constexpr int addr(int pencil, int slot) {
         return ALONG_X ? at(pencil, slot) : at(slot, pencil);
@@ -255,12 +257,14 @@ struct PlaneLayout {
 // conflict-free.  A job's 8 lanes touch slots 8m + (t ^ c) of its pencils
 // (m fixed or t-dependent): along a row those are 8 distinct banks, down a
 // column rows r with 2r mod 32 spread over 8 distinct even banks.  So
-//   x pass (pencils = rows): the 4 jobs of a half-wavefront take rows
-//     e, e+4, e+8, e+12 (2r = 2e + 0, 8, 16, 24 mod 32), the 2 of a
-//     quarter-wavefront rows 4 apart (8 banks apart mod 16);
-//   y pass (pencils = columns): the 4 jobs take columns c, c+1, c+16, c+17
-//     (odd/even and low/high halves of the 32 banks), the 2 of a
-//     quarter-wavefront c, c+1.
+//   x pass (pencils = rows): a job takes rows r and r + 1 (r even: the
+//     second pencil one row on, an offset a ds instruction carries); the 4
+//     jobs of a half-wavefront take r = e, e+4, e+8, e+12 (2r = 2e + 0, 8,
+//     16, 24 mod 32), the 2 of a quarter-wavefront rows 4 apart (8 banks
+//     apart mod 16);
+//   y pass (pencils = columns): a job takes columns c and c + 64; the 4 jobs
+//     of a half-wavefront take c, c+1, c+16, c+17 (odd/even and low/high
+//     halves of the 32 banks), the 2 of a quarter-wavefront c, c+1.
 // 128 rows of 130 slots: 133 KB for a FP64 plane (the swizzled layout's
 // pitch 136 takes 139 KB).
 template <int P, bool ALONG_X>
@@ -268,8 +272,9 @@ struct PlaneLayoutB64 {
     static_assert(P % 32 == 2, "pitch 2 mod 32");
     __device__ static constexpr int pencil(int jj) {
         const int q = jj >> 2, i = jj & 3;
-        return ALONG_X ? 16 * (q >> 2) + (q & 3) + 4 * i : 32 * (q >> 3) + 2 * (q & 7) + (i & 1) + 16 * (i >> 1);
+        return ALONG_X ? 16 * (q >> 1) + 2 * (q & 1) + 4 * i : 32 * (q >> 3) + 2 * (q & 7) + (i & 1) + 16 * (i >> 1);
     }
+    static constexpr int pair_delta(int jobs) { return ALONG_X ? 1 : jobs; }
     __device__ static constexpr int at(int row, int col) { return row * P + col; }
     __device__ static constexpr int addr(int pencil, int slot) {
         return ALONG_X ? at(pencil, slot) : at(slot, pencil);
@@ -390,9 +395,9 @@ __device__ __forceinline__ void pencil_pairs(T* __restrict__ lds, const Lane& L,
         if (w < WORK) {
             const int g = Lay::pencil(w / N1);
 // slot = lv + cp: lv lane-dependent, cp a compile-time multiple of N1 (see
-// PlaneLayout::addr_split); pencil gb = g + JOBS sits at a constant offset
+// PlaneLayout::addr_split); pencil gb = g + Lay::pair_delta(JOBS) sits at a constant offset
 #define PA(lv, cp) lds[Lay::template addr_split<0>(g, (lv), (cp))]
-#define PB(lv, cp) lds[Lay::template addr_split<JOBS>(g, (lv), (cp))]
+#define PB(lv, cp) lds[Lay::template addr_split<Lay::pair_delta(JOBS)>(g, (lv), (cp))]
 
             // ---- step 1: pre-processing + N2-point DFTs over j2 (j = t + N1*j2)
             T yr[N2], yi[N2];
@@ -602,6 +607,17 @@ __device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const Lane& L,
     const int rA = opaque(pb + rowb((N - t) & (N - 1)));
     const int rB = opaque(pb + rowb(8 - t));   // f(8) = f(0): row 8(15 - j2) + 8 - t for every t
     const int rC = opaque(pb + rowb(S::posZ((N - 8 * t) & (N - 1))));
+    // A ds instruction carries a byte offset below 64 KiB.  When 8 row groups
+    // of the plane span more (FP64 columns), the accesses 8..15 row groups on
+    // go from bases set 8 groups further (one add each, once per pass) instead
+    // of an add per access.
+    constexpr bool HI = COLS && 15 * ROW8 > 65535;
+    const int b0h = HI ? opaque(pb + 8 * ROW8) : b0, rBh = HI ? opaque(rB + 8 * ROW8) : rB,
+              b1h = HI ? opaque(b1 + 8 * ROW8) : b1;
+    // lane base + c: c < 8 groups from `lo`, else from `hi` (8 groups on)
+    auto split = [&](int lo, int hi, int lane, int groups) {
+        return (HI && groups >= 8) ? hi + lane + ROW8 * (groups - 8) : lo + lane + ROW8 * groups;
+    };
     auto A = [&](int off) -> T& { return *reinterpret_cast<T*>(base + off); };
     auto B = [&](int off) -> T& { return *reinterpret_cast<T*>(base + off + PBO); };
 
@@ -609,8 +625,8 @@ __device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const Lane& L,
     T yr[N2], yi[N2];
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) {
-        const int o0 = b0 + H[0] + ROW8 * j2;
-        const int o1 = j2 == 0 ? rA : rB + ROW8 * (15 - j2);
+        const int o0 = split(b0, b0h, H[0], j2);
+        const int o1 = j2 == 0 ? rA : split(rB, rBh, 0, 15 - j2);
         const T a0 = A(o0), a1 = A(o1), c0 = B(o0), c1 = B(o1);
         const T sp = L.sine(j2);
         yr[j2] = fma(sp, a0 + a1, -a1);
@@ -641,7 +657,7 @@ __device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const Lane& L,
         T vr[N1], vi[N1];
 #pragma unroll
         for (int j1 = 0; j1 < N1; ++j1) {
-            const int o = b1 + H[j1] + ROW8 * 8 * r;
+            const int o = split(b1, b1h, H[j1], 8 * r);
             vr[j1] = A(o);
             vi[j1] = B(o);
         }
@@ -658,7 +674,7 @@ __device__ __forceinline__ void dst_pairs128(T* __restrict__ lds, const Lane& L,
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) {   // posZ(t + cz) = cz + (t ^ ((r + 2 k1) mod 8)), cz = 8 (r + 2 k1)
             const int m = r + 2 * k1;
-            const int o = b0 + H[m & 7] + ROW8 * m;
+            const int o = split(b0, b0h, H[m & 7], m);
             A(o) = zr[r * N1 + k1];
             B(o) = zi[r * N1 + k1];
         }
