@@ -11,19 +11,23 @@
 // the reference's per-step maxima line and, at exit, its per-phase step
 // times.  VTK snapshots step_%07d.vtk in the format of
 // velocity_plotter::vtk_out (src/velocity_plot.cpp:113-218, Cartesian
-// branch); PNG plots need plplot and are not produced.  --dry-run prints the
-// resolved parameters and exits without touching the GPU.
+// branch), and step_%07d.png with the six panels of velocity_plotter::plot
+// (velocity_png.hpp: filled 10-level maps, titles in tEXt chunks; the stream
+// functions solved on the GPU by the LaplRect drop-ins).  --dry-run prints
+// the resolved parameters and exits without touching the GPU.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #define FDMGPU_THROW 1
 #include "../include/fdm/config.h"
 #include "../include/fdm/ns_cube.h"
+#include "velocity_png.hpp"
 
 namespace {
 
@@ -58,17 +62,29 @@ int calc(const Config& c) {
     const int steps = c.get("ns", "steps", 1);
     const int plot_interval = c.get("plot", "interval", 100);
     const int png = c.get("plot", "png", 1), vtk = c.get("plot", "vtk", 0);
-    if (png) std::fprintf(stderr, "plot:png needs plplot: no PNG plots are written\n");
+    std::unique_ptr<fdmpng::VelocityPng> plot;
+    if (png)
+        plot.reset(new fdmpng::VelocityPng(ns.dx, ns.dy, ns.dz, ns.nx, ns.ny, ns.nz, ns.x1, ns.x2, ns.y1, ns.y2,
+                                           ns.z1, ns.z2));
     auto snapshot = [&]() {
         char name[64];
-        std::snprintf(name, sizeof name, "step_%07d.vtk", ns.time_index);
-        vtk_out(name, ns);
+        if (png) {
+            std::snprintf(name, sizeof name, "step_%07d.png", ns.time_index);
+            if (!plot->plot(name, ns.time_index * ns.dt, ns.u.get(), ns.v.get(), ns.w.get())) {
+                std::perror(name);
+                std::exit(1);
+            }
+        }
+        if (vtk) {
+            std::snprintf(name, sizeof name, "step_%07d.vtk", ns.time_index);
+            vtk_out(name, ns);
+        }
     };
-    if (vtk) snapshot();
+    if (png || vtk) snapshot();
     const auto t1 = std::chrono::steady_clock::now();
     for (int i = 0; i < steps; i++) {
         ns.step();
-        if ((i + 1) % plot_interval == 0 && vtk) snapshot();
+        if ((i + 1) % plot_interval == 0 && (png || vtk)) snapshot();
     }
     (void)ns.solver_stats();   // waits for the last step
     const auto t2 = std::chrono::steady_clock::now();
