@@ -532,6 +532,83 @@ __global__ void add_f32_to_f64(const BatchPtr<double> u, const float* __restrict
 // launch skips it, so the host never synchronises inside a solve.
 // first > 1 continues a refinement whose iterate (in ans) after `first - 1`
 // inner solves did not reach the tolerance; last = the final inner solve.
+// Periodic cubes (every axis periodic; singular operator): the zero-mean
+// solution of A u = f - mean f, as the reference's periodic solve returns it
+// (its (0,0,0) mode is zeroed, src/lapl_cube.cpp:82-84).
+//   once per call: mean f per problem (fsum + fmean)
+//   solve 1:      r32 = f - mean f, c = solve32(r32), u = c - mean c
+//   solve i >= 2: c = solve32(r32), u += c - mean c
+//   after each:   r = (f - mean f) - A u (wrap-around stencil), ||r|| /
+//                 ||f - mean f|| -> per-problem flag; r32 = r for the next
+// The inner solve is the plan's FP32 periodic solve in place on r32; the
+// per-problem means of f and of the correction live in the FP32 iterate
+// scratch u32, which the periodic path does not otherwise use.
+int solve_refine_periodic(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const double* rhs, Tab rhs_tab,
+                          int count, hipStream_t s, int first, int last) {
+    const long elems = (long)P->nx * P->ny * P->nz;
+    PeriodicResidualArgs pa{};
+    pa.f = rhs;
+    pa.f_tab = rhs_tab;
+    pa.u = ans;
+    pa.u_tab = ans_tab;
+    pa.fmean = (double*)W.u32;
+    pa.cmean = (double*)W.u32 + count;   // u32 holds 4 elems >= 16 bytes per problem (n >= 2 per axis)
+    pa.r32 = W.r32;
+    pa.partial = W.partial;
+    pa.nx = P->nx;
+    pa.ny = P->ny;
+    pa.nz = P->nz;
+    pa.nblk = (int)std::max<long>(1, std::min<long>(residual_partials(P->nx, P->ny, P->nz, true), elems));
+    pa.ix = 1.0 / (P->hx_op * P->hx_op);
+    pa.iy = 1.0 / (P->dy * P->dy);
+    pa.iz = 1.0 / (P->hz_op * P->hz_op);
+    pa.bs = elems;
+    pa.batch = count;
+    pa.tol = P->rtol;
+    pa.done = W.done;
+    pa.iters = W.iters;
+    pa.resid = W.resid;
+    pa.stream = s;
+    {
+        Timed tm(P, FDMGPU_K_OTHER, s);
+        HIPCHK(launch_periodic_fmean(pa));
+    }
+    if (first > 1) {   // the current iterate: flags, and r32 for the next solve
+        pa.check = true;
+        pa.it = first - 1;
+        Timed tm(P, FDMGPU_K_RESIDUAL_RHS, s);
+        HIPCHK(launch_periodic_residual(pa));
+    }
+    for (int it = first; it <= last; ++it) {
+        if (it == 1) {   // r32 = f - mean f
+            pa.u = nullptr;
+            pa.u_tab = nullptr;
+            pa.r32 = W.r32;
+            pa.check = false;
+            Timed tm(P, FDMGPU_K_RESIDUAL_RHS, s);
+            HIPCHK(launch_periodic_residual(pa));
+        }
+        int rc = stage_fwd<float, float>(P, W.r32, nullptr, W.r32, nullptr, count, s, W.done, FDMGPU_K_XY_FWD_R32);
+        if (rc) return rc;
+        rc = stage_tridiag<float>(P, W.r32, nullptr, count, s, W.done);
+        if (rc) return rc;
+        rc = stage_inv<float, float>(P, W.r32, nullptr, W.r32, nullptr, count, s, false, W.done);
+        if (rc) return rc;
+        {
+            Timed tm(P, FDMGPU_K_OTHER, s);
+            HIPCHK(launch_periodic_update(pa, ans, ans_tab, W.r32, it == 1));
+        }
+        pa.u = ans;
+        pa.u_tab = ans_tab;
+        pa.r32 = it < last ? W.r32 : nullptr;
+        pa.check = true;
+        pa.it = it;
+        Timed tm(P, pa.r32 ? FDMGPU_K_RESIDUAL_RHS : FDMGPU_K_RESIDUAL, s);
+        HIPCHK(launch_periodic_residual(pa));
+    }
+    return FDMGPU_OK;
+}
+
 int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const double* rhs, Tab rhs_tab,
                  int count, hipStream_t s, int first, int last) {
     int rc = work_reserve(P, W, count);
@@ -554,6 +631,7 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
         Timed tm(P, FDMGPU_K_OTHER, s);
         HIPCHK(launch_fill_int(W.done, W.iters, count, 0, s));
     }
+    if (P->periodic) return solve_refine_periodic(P, W, ans, ans_tab, rhs, rhs_tab, count, s, first, last);
     ResidualArgs ra{};
     ra.f = rhs;
     ra.u = ans;
@@ -857,8 +935,6 @@ int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double
     const int lgx = ilog2_exact(nx + e), lgy = ilog2_exact(ny + e), lgz = ilog2_exact(nz + e);
     if (lgx < 0 || lgy < 0 || lgz < 0) return FDMGPU_ERR_INVALID_SIZE;
     if (lgx > 9 || lgy > 9 || lgz > 9 || lgx < 1 || lgy < 1 || lgz < 1) return FDMGPU_ERR_UNSUPPORTED;
-    // the residual of a periodic (singular) operator is not implemented for refinement
-    if (per && precision == FDMGPU_F32_REFINE64) return FDMGPU_ERR_UNSUPPORTED;
     fdmgpu_plan_t* P = new (std::nothrow) fdmgpu_plan_t();
     if (!P) return FDMGPU_ERR_OUT_OF_MEMORY;
     int rc = common_init(P, precision);
@@ -872,8 +948,9 @@ int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double
     P->dx = dx; P->dy = dy; P->dz = dz; P->lx = lx; P->ly = ly; P->lz = lz;
     if (precision == FDMGPU_F32_REFINE64) {
         // refinement converges to A^{-1} f; the reference returns g A^{-1} f with
-        // g = prod_a h_a N_a / l_a, which is 1 for the usual l = (n+1) h geometry
-        const double g = dx * (nx + 1) / lx * dy * (ny + 1) / ly * dz * (nz + 1) / lz;
+        // g = prod_a h_a N_a / l_a (N = n + 1 Dirichlet, n periodic), which is 1
+        // for the usual l = N h geometry
+        const double g = dx * (nx + e) / lx * dy * (ny + e) / ly * dz * (nz + e) / lz;
         if (std::fabs(g - 1.0) > 1e-13) { free_plan(P); return FDMGPU_ERR_UNSUPPORTED; }
     }
     // eigenvalue aliasing of the reference (src/lapl_cube.cpp:162,171)

@@ -408,6 +408,165 @@ long residual_partials(int nx, int ny, int nz, bool three) {
     return b < kResMaxBlocks ? b : kResMaxBlocks;
 }
 
+// ---------------------------------------------------------------------------
+// Periodic cubes (every axis periodic).  The operator is singular (it maps
+// constants to zero) and the solve returns the zero-mean solution of
+// A u = f - mean(f): the reference zeroes the (0,0,0) mode
+// (src/lapl_cube.cpp:82-84).  Refinement therefore works on the projected
+// residual r = (f - mean f) - A u, with A the 7-point stencil wrapping around
+// every axis, and certifies ||r|| / ||f - mean f||.  mean f comes from a
+// first pass; subtracting it before the squares keeps a large mean from
+// swamping the small projected residual.  Each launch covers nblk contiguous
+// chunks of a problem's points (block b: [b ch, (b+1) ch), ch = ceil(N/nblk))
+// and reduces in a fixed order, so a result does not depend on the batch.
+// ---------------------------------------------------------------------------
+constexpr int kPerBlock = 256;
+
+__device__ __forceinline__ void block_sum2_fixed(double& a, double& b, double* sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = a;
+    sh[kPerBlock + tid] = b;
+    __syncthreads();
+#pragma unroll
+    for (int s = kPerBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            sh[tid] += sh[tid + s];
+            sh[kPerBlock + tid] += sh[kPerBlock + tid + s];
+        }
+        __syncthreads();
+    }
+    a = sh[0];
+    b = sh[kPerBlock];
+}
+
+// partial[p][b][0] = sum of f over chunk b of problem p (in FP64)
+template <typename T>
+__global__ __launch_bounds__(kPerBlock) void fsum_kernel(const BatchPtr<const T> f, double* __restrict__ partial,
+                                                         const long n, const int nblk, const int* __restrict__ done) {
+    __shared__ double sh[2 * kPerBlock];
+    const int p = blockIdx.y, b = blockIdx.x;
+    if (done && done[p]) return;
+    const long ch = (n + nblk - 1) / nblk, lo = b * ch, hi = lo + ch < n ? lo + ch : n;
+    const auto fp = as_global(f(p));
+    double s = 0.0, unused = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kPerBlock) s += (double)fp[i];
+    block_sum2_fixed(s, unused, sh);
+    if (threadIdx.x == 0) {
+        partial[((long)p * nblk + b) * 2] = s;
+        partial[((long)p * nblk + b) * 2 + 1] = 0.0;
+    }
+}
+
+// fmean[p] = (sum of the nblk chunk sums, in chunk order) / n
+__global__ __launch_bounds__(kPerBlock) void fmean_kernel(const double* __restrict__ partial, const int nblk,
+                                                          const long n, double* __restrict__ fmean,
+                                                          const int* __restrict__ done) {
+    __shared__ double sh[2 * kPerBlock];
+    const int p = blockIdx.x;
+    if (done && done[p]) return;
+    double s = 0.0, unused = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kPerBlock) s += partial[((long)p * nblk + b) * 2];
+    block_sum2_fixed(s, unused, sh);
+    if (threadIdx.x == 0) fmean[p] = s / (double)n;
+}
+
+// r = (f - fmean) - A u (u == null: u = 0); r32 = (float) r when not null;
+// partial[p][b] = (sum r^2, sum (f - fmean)^2) over chunk b
+__global__ __launch_bounds__(kPerBlock) void residual_periodic_kernel(
+    const BatchPtr<const double> f, const BatchPtr<const double> u, const double* __restrict__ fmean,
+    float* __restrict__ r32, double* __restrict__ partial, const int nx, const int ny, const int nz,
+    const double ix, const double iy, const double iz, const long bs, const int nblk, const int* __restrict__ done) {
+    __shared__ double sh[2 * kPerBlock];
+    const int p = blockIdx.y, b = blockIdx.x;
+    if (done && done[p]) return;
+    const long pl = (long)nx * ny, n = pl * nz;
+    const long ch = (n + nblk - 1) / nblk, lo = b * ch, hi = lo + ch < n ? lo + ch : n;
+    const auto fp = as_global(f(p));
+    const double fm = fmean[p];
+    const bool hu = u.base || u.tab;
+    const double* up = hu ? u(p) : nullptr;
+    double rr = 0.0, ff = 0.0;
+    for (long c = lo + threadIdx.x; c < hi; c += kPerBlock) {
+        const double fv = fp[c] - fm;
+        double au = 0.0;
+        if (hu) {
+            const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / pl);
+            const long row = c - x, plane = c - (long)y * nx - x;
+            const double uc = up[c];
+            const double xm = up[row + (x == 0 ? nx - 1 : x - 1)], xp = up[row + (x == nx - 1 ? 0 : x + 1)];
+            const double ym = up[plane + (long)(y == 0 ? ny - 1 : y - 1) * nx + x];
+            const double yp = up[plane + (long)(y == ny - 1 ? 0 : y + 1) * nx + x];
+            const double zm = up[(long)(z == 0 ? nz - 1 : z - 1) * pl + (c - (long)z * pl)];
+            const double zp = up[(long)(z == nz - 1 ? 0 : z + 1) * pl + (c - (long)z * pl)];
+            au = (xm - 2.0 * uc + xp) * ix + (ym - 2.0 * uc + yp) * iy + (zm - 2.0 * uc + zp) * iz;
+        }
+        const double r = fv - au;
+        rr += r * r;
+        ff += fv * fv;
+        if (r32) r32[p * bs + c] = (float)r;
+    }
+    block_sum2_fixed(rr, ff, sh);
+    if (threadIdx.x == 0) {
+        partial[((long)p * nblk + b) * 2] = rr;
+        partial[((long)p * nblk + b) * 2 + 1] = ff;
+    }
+}
+
+// u = c - mean c (assign) or u += c - mean c, skipping converged problems.
+// The FP32 inner solve zeroes the (0,0,0) mode only up to its rounding, and A
+// cannot see a constant, so the refinement would keep the first solve's mean
+// error; taking each correction's mean out keeps u zero-mean as the FP64
+// solve returns it (and leaves every residual unchanged).
+__global__ void update_f64_kernel(const BatchPtr<double> u, const float* __restrict__ c, const double* __restrict__ cmean,
+                                  const long n, const long bs, const bool assign, const int* __restrict__ done) {
+    const int p = blockIdx.y;
+    if (done && done[p]) return;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const auto up = as_global(u(p));
+    const double v = (double)c[p * bs + i] - cmean[p];
+    up[i] = assign ? v : up[i] + v;
+}
+
+template <typename T>
+static hipError_t launch_mean(const BatchPtr<const T>& f, double* out, const PeriodicResidualArgs& a) {
+    const long n = (long)a.nx * a.ny * a.nz;
+    hipLaunchKernelGGL(fsum_kernel<T>, dim3(a.nblk, a.batch), dim3(kPerBlock), 0, a.stream, f, a.partial, n, a.nblk,
+                       a.done);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fmean_kernel, dim3(a.batch), dim3(kPerBlock), 0, a.stream, (const double*)a.partial, a.nblk,
+                       n, out, a.done);
+    return hipGetLastError();
+}
+
+hipError_t launch_periodic_fmean(const PeriodicResidualArgs& a) {
+    return launch_mean(BatchPtr<const double>{a.f, a.f_tab, a.bs}, a.fmean, a);
+}
+
+hipError_t launch_periodic_residual(const PeriodicResidualArgs& a) {
+    const BatchPtr<const double> f{a.f, a.f_tab, a.bs}, u{a.u, a.u_tab, a.bs};
+    hipLaunchKernelGGL(residual_periodic_kernel, dim3(a.nblk, a.batch), dim3(kPerBlock), 0, a.stream, f, u,
+                       (const double*)a.fmean, a.r32, a.partial, a.nx, a.ny, a.nz, a.ix, a.iy, a.iz, a.bs, a.nblk,
+                       a.done);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !a.check) return e;
+    hipLaunchKernelGGL(check_kernel, dim3(a.batch), dim3(kCheckBlock), 0, a.stream, (const double*)a.partial, a.nblk,
+                       a.tol, a.done, a.iters, a.resid, a.it);
+    return hipGetLastError();
+}
+
+hipError_t launch_periodic_update(const PeriodicResidualArgs& a, double* u, const void* const* u_tab, const float* c,
+                                  bool assign) {
+    hipError_t e = launch_mean(BatchPtr<const float>{c, nullptr, a.bs}, a.cmean, a);
+    if (e != hipSuccess) return e;
+    const long n = (long)a.nx * a.ny * a.nz;
+    const dim3 g((unsigned)((n + 255) / 256), a.batch);
+    hipLaunchKernelGGL(update_f64_kernel, g, dim3(256), 0, a.stream, BatchPtr<double>{u, u_tab, a.bs}, c,
+                       (const double*)a.cmean, n, a.bs, assign, a.done);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_int(int* a, int* b, int n, int v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n, v);

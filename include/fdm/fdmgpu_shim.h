@@ -25,12 +25,13 @@ inline int& double_mode_ref() {
 }
 inline void set_double_mode(int precision) { double_mode_ref() = precision; }
 
-// Periodic solvers have a singular operator (zero mode), which the FP64
-// refinement residual does not handle: they always solve in plain FP64.
+// Periodic 2-D solvers (LaplRect, LaplRectFFT2) always solve in plain FP64:
+// the projected-residual refinement of a singular operator exists for
+// periodic cubes only (refine_periodic = true).
 template <typename T>
-inline int precision_for(bool periodic = false) {
+inline int precision_for(bool periodic = false, bool refine_periodic = false) {
     if (sizeof(T) == sizeof(float)) return FDMGPU_F32;
-    return periodic ? FDMGPU_F64 : double_mode_ref();
+    return periodic && !refine_periodic ? FDMGPU_F64 : double_mode_ref();
 }
 
 // The reference reports failures through verify(): message on stderr, then

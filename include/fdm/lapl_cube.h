@@ -51,7 +51,7 @@ public:
         const int mask = (has_tensor_flag(xflag, tensor_flag::periodic) ? FDMGPU_PERIODIC_X : 0) |
                          (has_tensor_flag(yflag, tensor_flag::periodic) ? FDMGPU_PERIODIC_Y : 0) |
                          (has_tensor_flag(zflag, tensor_flag::periodic) ? FDMGPU_PERIODIC_Z : 0);
-        FDMGPU_VERIFY(fdmgpu_plan_cube(&plan_, gpu::precision_for<T>(mask != 0), mask, dx, dy, dz, lx, ly, lz, nx, ny, nz));
+        FDMGPU_VERIFY(fdmgpu_plan_cube(&plan_, gpu::precision_for<T>(mask != 0, true), mask, dx, dy, dz, lx, ly, lz, nx, ny, nz));
     }
 
     LaplCube(const LaplCube&) = delete;

@@ -54,7 +54,11 @@ typedef struct fdmgpu_plan_t* fdmgpu_plan;
  * (src/lapl_cube.h:58-100) and its init_lm (src/lapl_cube.cpp:145-172).
  * Transform lengths nx+1, ny+1, nz+1 (Dirichlet) or nx, ny, nz (periodic) must
  * be powers of two.  periodic_mask: 0 (tensor_flags<>) or 7 (periodic^3), the
- * reference's instantiations (src/lapl_cube.cpp:174-182). */
+ * reference's instantiations (src/lapl_cube.cpp:174-182).  Periodic cubes
+ * return the zero-mean solution of A u = f - mean(f) (the reference zeroes
+ * the (0,0,0) mode, src/lapl_cube.cpp:82-84); with FDMGPU_F32_REFINE64 the
+ * refinement certifies ||(f - mean f) - A u|| / ||f - mean f|| (requires
+ * l = n h per axis). */
 int fdmgpu_plan_cube(fdmgpu_plan* plan, int precision, int periodic_mask, double dx, double dy, double dz,
                      double lx, double ly, double lz, int nx, int ny, int nz);
 

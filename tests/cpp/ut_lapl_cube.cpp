@@ -202,9 +202,15 @@ int main() {
     e = cube_analytic<float>(31, FDMGPU_F64);
     std::printf("LaplCube<float> 31^3 analytic: err %.3e\n", e);
     CHECK(e < 1e-4, "cube float err %g", e);
+    fdm::gpu::set_double_mode(FDMGPU_F64);
     e = cube_periodic<double>(32);
     std::printf("LaplCube<double, periodic^3> 32^3 analytic: err %.3e\n", e);
     CHECK(e < 1e-2, "cube periodic double err %g", e);
+    fdm::gpu::set_double_mode(FDMGPU_F32_REFINE64);
+    const double er = cube_periodic<double>(32);
+    std::printf("LaplCube<double, periodic^3> 32^3 analytic (refine): err %.3e\n", er);
+    CHECK(std::fabs(er - e) <= 1e-9 * (1 + e), "cube periodic refine err %g vs FP64 %g", er, e);
+    fdm::gpu::set_double_mode(FDMGPU_F64);
     e = cube_periodic<float>(32);
     std::printf("LaplCube<float, periodic^3> 32^3 analytic: err %.3e\n", e);
     CHECK(e < 1e-2, "cube periodic float err %g", e);

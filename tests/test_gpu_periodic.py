@@ -154,6 +154,65 @@ def test_periodic_errors(fdm):
     with pytest.raises(fdm.FdmGpuError) as e:
         fdm.LaplCube(0.1, 0.1, 0.1, 1, 1, 1, 6, 8, 8, periodic=True)     # n = 6 is not a power of two
     assert e.value.code == fdm.ERR_INVALID_SIZE
-    with pytest.raises(fdm.FdmGpuError) as e:                             # FP64 residual of a singular operator
-        fdm.LaplCube(0.1, 0.1, 0.1, 1.6, 1.6, 1.6, 16, 16, 16, precision="refine", periodic=True)
+    with pytest.raises(fdm.FdmGpuError) as e:      # refinement needs l = n h (the solve's scale 1)
+        fdm.LaplCube(0.1, 0.1, 0.1, 1.7, 1.6, 1.6, 16, 16, 16, precision="refine", periodic=True)
     assert e.value.code == fdm.ERR_UNSUPPORTED
+    with pytest.raises(fdm.FdmGpuError) as e:      # periodic LaplRect: no projected refinement
+        fdm.LaplRect(0.1, 0.1, 1.7, 1.6, 16, 16, precision="refine", yperiodic=True)
+    assert e.value.code == fdm.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (64, 64, 64), (128, 128, 128), (16, 32, 8),
+                                   (8, 8, 256), (256, 16, 8)])
+def test_cube_periodic_refine(fdm, shape):
+    """FP32+refine on periodic cubes: the zero-mean solution of A u = f - mean f
+    (the reference's periodic solve, src/lapl_cube.cpp:82-84) with f of non-zero
+    mean.  Bars: device-reported projected residual and the host's
+    ||(f - mean f) - A u|| / ||f - mean f|| (FP64, numpy) <= 1e-10; within 1e-9
+    (relative L2) of the FP64 oracle; mean(u) ~ 0."""
+    nx, ny, nz = shape
+    say("periodic cube refine", shape)
+    g = periodic_geom(nx, ny, nz, aniso=(nx != ny or ny != nz))
+    f = O.field(15, (nz, ny, nx)) + 0.75
+    ref = O.lapl_cube(f, *g, periodic=True)
+    s = fdm.LaplCube(*g, nx, ny, nz, precision="refine", periodic=True)
+    u = np.zeros_like(f)
+    s.solve(u, f)
+    st = s.stats()
+    assert 1 <= st.iterations <= 4 and 0 <= st.rel_residual <= 1e-10, (st.iterations, st.rel_residual)
+    fp = f - f.mean()
+    r = fp - lapl_periodic(u, s_hx(g, shape), g[1], s_hz(g, shape))
+    assert np.linalg.norm(r) / np.linalg.norm(fp) <= 1e-10
+    assert rel2(u, ref) <= 1e-9
+    assert abs(u.mean()) <= 1e-9 * np.abs(u).max()
+
+
+def s_hx(g, shape):
+    # the operator the solve inverts takes lambda_x from y when nx == ny
+    # (reference init_lm eigenvalue aliasing, src/lapl_cube.cpp:162)
+    return g[1] if shape[0] == shape[1] else g[0]
+
+
+def s_hz(g, shape):
+    return g[1] if shape[2] == shape[1] else g[2]
+
+
+def test_cube_periodic_refine_batch_in_place(fdm):
+    """Three problems back to back, solved in place (ans == rhs): each equals
+    its solo refine solve bitwise and sits within 1e-9 of the FP64 oracle."""
+    from fdm_amd import hip
+    n, B = 32, 3
+    say("periodic refine batch in place", n, B)
+    g = periodic_geom(n, n, n)
+    F = np.stack([O.field(30 + i, (n, n, n)) + 0.5 * i for i in range(B)])
+    s = fdm.LaplCube(*g, n, n, n, precision="refine", periodic=True)
+    d = hip.DeviceArray.from_host(F)
+    st = hip.Stream()
+    s.solve_batch(d, d, st)
+    st.synchronize()
+    U = d.download()
+    for i in range(B):
+        solo = np.zeros_like(F[i])
+        s.solve(solo, F[i])
+        assert np.array_equal(U[i], solo), i
+        assert rel2(U[i], O.lapl_cube(F[i], *g, periodic=True)) <= 1e-9
