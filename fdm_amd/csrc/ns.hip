@@ -28,6 +28,9 @@ namespace {
 struct Grid {
     int nx, ny, nz;
     double dx, dy, dz, U0, Re, dt;
+    // constant divisors of the stencils as reciprocals (set on the host):
+    // 1/(Re h^2) per axis, 1/h per axis, dt/h per axis, 1/dt, 1/h^2 per axis
+    double cx, cy, cz, idx, idy, idz, tdx, tdy, tdz, idt, idx2, idy2, idz2;
     // element offsets of (i, k, j) in each array
     __device__ long u(int i, int k, int j) const { return ((long)i * (ny + 2) + k) * (nx + 3) + (j + 1); }
     __device__ long v(int i, int k, int j) const { return ((long)i * (ny + 3) + (k + 1)) * (nx + 2) + j; }
@@ -151,43 +154,46 @@ __global__ __launch_bounds__(kTileX * kTileY) void ns_fgh_kernel(Grid g, Fields 
     const auto U = as_global((const double*)f.u);
     const auto V = as_global((const double*)f.v);
     const auto W = as_global((const double*)f.w);
-    const double Re = g.Re, dt = g.dt, dx = g.dx, dy = g.dy, dz = g.dz;
-    const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+    // every value F, G and H use, loaded once before any store (F, G, H
+    // could alias u, v, w for all the compiler knows: loads after a store
+    // would be issued again).  Neighbours outside an array's index range
+    // (k - 1 < 0 for u and w, i - 1 < 0 for u and v, j - 1 < 0 for v and w)
+    // are clamped into it; their values are used only where the unclamped
+    // index is in range (F: i, k >= 1; G: i, j >= 1; H: k, j >= 1).
+    const int im = i > 0 ? i - 1 : 0, km = k > 0 ? k - 1 : 0, jm = j > 0 ? j - 1 : 0;
+    const double u0 = U[g.u(i, k, j)], uxp = U[g.u(i, k, j + 1)], uxm = U[g.u(i, k, j - 1)];
+    const double uyp = U[g.u(i, k + 1, j)], uym = U[g.u(i, km, j)];
+    const double uzp = U[g.u(i + 1, k, j)], uzm = U[g.u(im, k, j)];
+    const double uypxm = U[g.u(i, k + 1, j - 1)], uzpxm = U[g.u(i + 1, k, j - 1)];
+    const double v0 = V[g.v(i, k, j)], vxp = V[g.v(i, k, j + 1)], vxm = V[g.v(i, k, jm)];
+    const double vyp = V[g.v(i, k + 1, j)], vym = V[g.v(i, k - 1, j)];
+    const double vzp = V[g.v(i + 1, k, j)], vzm = V[g.v(im, k, j)];
+    const double vymxp = V[g.v(i, k - 1, j + 1)], vzpym = V[g.v(i + 1, k - 1, j)];
+    const double w0 = W[g.w(i, k, j)], wxp = W[g.w(i, k, j + 1)], wxm = W[g.w(i, k, jm)];
+    const double wyp = W[g.w(i, k + 1, j)], wym = W[g.w(i, km, j)];
+    const double wzp = W[g.w(i + 1, k, j)], wzm = W[g.w(i - 1, k, j)];
+    const double wzmxp = W[g.w(i - 1, k, j + 1)], wzmyp = W[g.w(i - 1, k + 1, j)];
+    const double dt = g.dt;
     if (i >= 1 && k >= 1) {
-        const double uc = U[g.u(i, k, j)];
         as_global(f.F)[g.F(i, k, j)] =
-            uc + dt * ((U[g.u(i, k, j + 1)] - 2 * uc + U[g.u(i, k, j - 1)]) / Re / dx2 +
-                       (U[g.u(i, k + 1, j)] - 2 * uc + U[g.u(i, k - 1, j)]) / Re / dy2 +
-                       (U[g.u(i + 1, k, j)] - 2 * uc + U[g.u(i - 1, k, j)]) / Re / dz2 -
-                       (sq(0.5 * (uc + U[g.u(i, k, j + 1)])) - sq(0.5 * (U[g.u(i, k, j - 1)] + uc))) / dx -
-                       0.25 * ((uc + U[g.u(i, k + 1, j)]) * (V[g.v(i, k, j + 1)] + V[g.v(i, k, j)]) -
-                               (U[g.u(i, k - 1, j)] + uc) * (V[g.v(i, k - 1, j + 1)] + V[g.v(i, k - 1, j)])) / dy -
-                       0.25 * ((uc + U[g.u(i + 1, k, j)]) * (W[g.w(i, k, j + 1)] + W[g.w(i, k, j)]) -
-                               (U[g.u(i - 1, k, j)] + uc) * (W[g.w(i - 1, k, j + 1)] + W[g.w(i - 1, k, j)])) / dz);
+            u0 + dt * ((uxp - 2 * u0 + uxm) * g.cx + (uyp - 2 * u0 + uym) * g.cy + (uzp - 2 * u0 + uzm) * g.cz -
+                       (sq(0.5 * (u0 + uxp)) - sq(0.5 * (uxm + u0))) * g.idx -
+                       0.25 * ((u0 + uyp) * (vxp + v0) - (uym + u0) * (vymxp + vym)) * g.idy -
+                       0.25 * ((u0 + uzp) * (wxp + w0) - (uzm + u0) * (wzmxp + wzm)) * g.idz);
     }
     if (i >= 1 && j >= 1) {
-        const double vc = V[g.v(i, k, j)];
         as_global(f.G)[g.G(i, k, j)] =
-            vc + dt * ((V[g.v(i, k, j + 1)] - 2 * vc + V[g.v(i, k, j - 1)]) / Re / dx2 +
-                       (V[g.v(i, k + 1, j)] - 2 * vc + V[g.v(i, k - 1, j)]) / Re / dy2 +
-                       (V[g.v(i + 1, k, j)] - 2 * vc + V[g.v(i - 1, k, j)]) / Re / dz2 -
-                       (sq(0.5 * (vc + V[g.v(i, k + 1, j)])) - sq(0.5 * (V[g.v(i, k - 1, j)] + vc))) / dy -
-                       0.25 * ((U[g.u(i, k, j)] + U[g.u(i, k + 1, j)]) * (V[g.v(i, k, j + 1)] + vc) -
-                               (U[g.u(i, k, j - 1)] + U[g.u(i, k + 1, j - 1)]) * (vc + V[g.v(i, k, j - 1)])) / dx -
-                       0.25 * ((W[g.w(i, k, j)] + W[g.w(i, k + 1, j)]) * (vc + V[g.v(i + 1, k, j)]) -
-                               (W[g.w(i - 1, k, j)] + W[g.w(i - 1, k + 1, j)]) * (V[g.v(i - 1, k, j)] + vc)) / dz);
+            v0 + dt * ((vxp - 2 * v0 + vxm) * g.cx + (vyp - 2 * v0 + vym) * g.cy + (vzp - 2 * v0 + vzm) * g.cz -
+                       (sq(0.5 * (v0 + vyp)) - sq(0.5 * (vym + v0))) * g.idy -
+                       0.25 * ((u0 + uyp) * (vxp + v0) - (uxm + uypxm) * (v0 + vxm)) * g.idx -
+                       0.25 * ((w0 + wyp) * (v0 + vzp) - (wzm + wzmyp) * (vzm + v0)) * g.idz);
     }
     if (k >= 1 && j >= 1) {
-        const double wc = W[g.w(i, k, j)];
         as_global(f.H)[g.H(i, k, j)] =
-            wc + dt * ((W[g.w(i, k, j + 1)] - 2 * wc + W[g.w(i, k, j - 1)]) / Re / dx2 +
-                       (W[g.w(i, k + 1, j)] - 2 * wc + W[g.w(i, k - 1, j)]) / Re / dy2 +
-                       (W[g.w(i + 1, k, j)] - 2 * wc + W[g.w(i - 1, k, j)]) / Re / dz2 -
-                       (sq(0.5 * (W[g.w(i + 1, k, j)] + wc)) - sq(0.5 * (W[g.w(i - 1, k, j)] + wc))) / dz -
-                       0.25 * ((U[g.u(i + 1, k, j)] + U[g.u(i, k, j)]) * (W[g.w(i, k, j + 1)] + wc) -
-                               (U[g.u(i + 1, k, j - 1)] + U[g.u(i, k, j - 1)]) * (wc + W[g.w(i, k, j - 1)])) / dx -
-                       0.25 * ((wc + W[g.w(i, k + 1, j)]) * (V[g.v(i, k, j)] + V[g.v(i + 1, k, j)]) -
-                               (W[g.w(i, k - 1, j)] + wc) * (V[g.v(i, k - 1, j)] + V[g.v(i + 1, k - 1, j)])) / dy);
+            w0 + dt * ((wxp - 2 * w0 + wxm) * g.cx + (wyp - 2 * w0 + wym) * g.cy + (wzp - 2 * w0 + wzm) * g.cz -
+                       (sq(0.5 * (wzp + w0)) - sq(0.5 * (wzm + w0))) * g.idz -
+                       0.25 * ((uzp + u0) * (wxp + w0) - (uzpxm + uxm) * (w0 + wxm)) * g.idx -
+                       0.25 * ((w0 + wyp) * (v0 + vzp) - (wym + w0) * (vym + vzpym)) * g.idy);
     }
 }
 
@@ -204,18 +210,17 @@ __global__ __launch_bounds__(kTileX * kTileY) void ns_rhs_kernel(Grid g, Fields 
     const auto Ga = as_global((const double*)f.G);
     const auto Ha = as_global((const double*)f.H);
     const auto P = as_global((const double*)f.p);
-    const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy, dz2 = g.dz * g.dz;
-    double r = ((Fa[g.F(i, k, j)] - Fa[g.F(i, k, j - 1)]) / g.dx + (Ga[g.G(i, k, j)] - Ga[g.G(i, k - 1, j)]) / g.dy +
-                (Ha[g.H(i, k, j)] - Ha[g.H(i - 1, k, j)]) / g.dz) /
-               g.dt;
+    double r = ((Fa[g.F(i, k, j)] - Fa[g.F(i, k, j - 1)]) * g.idx + (Ga[g.G(i, k, j)] - Ga[g.G(i, k - 1, j)]) * g.idy +
+                (Ha[g.H(i, k, j)] - Ha[g.H(i - 1, k, j)]) * g.idz) *
+               g.idt;
     // the pressure's boundary values, read only at the points next to a face
     // (p has a ghost layer on every side, so every neighbour is in the array)
-    if (i <= 1) r -= P[g.p(i - 1, k, j)] / dz2;
-    if (k <= 1) r -= P[g.p(i, k - 1, j)] / dy2;
-    if (j <= 1) r -= P[g.p(i, k, j - 1)] / dx2;
-    if (j >= g.nx) r -= P[g.p(i, k, j + 1)] / dx2;
-    if (k >= g.ny) r -= P[g.p(i, k + 1, j)] / dy2;
-    if (i >= g.nz) r -= P[g.p(i + 1, k, j)] / dz2;
+    if (i <= 1) r -= P[g.p(i - 1, k, j)] * g.idz2;
+    if (k <= 1) r -= P[g.p(i, k - 1, j)] * g.idy2;
+    if (j <= 1) r -= P[g.p(i, k, j - 1)] * g.idx2;
+    if (j >= g.nx) r -= P[g.p(i, k, j + 1)] * g.idx2;
+    if (k >= g.ny) r -= P[g.p(i, k + 1, j)] * g.idy2;
+    if (i >= g.nz) r -= P[g.p(i + 1, k, j)] * g.idz2;
     as_global(f.RHS)[g.x(i, k, j)] = r;
 }
 
@@ -237,9 +242,9 @@ __global__ __launch_bounds__(kTileX * kTileY) void ns_update_kernel(Grid g, Fiel
     const double xj = X[g.x(i, k, j < g.nx ? j + 1 : j)], xk = X[g.x(i, k < g.ny ? k + 1 : k, j)];
     const double xi = X[g.x(i < g.nz ? i + 1 : i, k, j)];
     const double fu = f.F[g.F(i, k, j)], fv = f.G[g.G(i, k, j)], fw = f.H[g.H(i, k, j)];
-    if (j < g.nx) as_global(f.u)[g.u(i, k, j)] = fu - g.dt / g.dx * (xj - xc);
-    if (k < g.ny) as_global(f.v)[g.v(i, k, j)] = fv - g.dt / g.dy * (xk - xc);
-    if (i < g.nz) as_global(f.w)[g.w(i, k, j)] = fw - g.dt / g.dz * (xi - xc);
+    if (j < g.nx) as_global(f.u)[g.u(i, k, j)] = fu - g.tdx * (xj - xc);
+    if (k < g.ny) as_global(f.v)[g.v(i, k, j)] = fv - g.tdy * (xk - xc);
+    if (i < g.nz) as_global(f.w)[g.w(i, k, j)] = fw - g.tdz * (xi - xc);
     as_global(f.p)[g.p(i, k, j)] = xc;
 }
 
@@ -425,6 +430,11 @@ int fdmgpu_ns_create(fdmgpu_ns* ns, int solver_precision, int nx, int ny, int nz
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.dx = (x2 - x1) / nx; g.dy = (y2 - y1) / ny; g.dz = (z2 - z1) / nz;
     g.U0 = u0; g.Re = Re; g.dt = dt;
+    g.cx = 1.0 / (Re * g.dx * g.dx); g.cy = 1.0 / (Re * g.dy * g.dy); g.cz = 1.0 / (Re * g.dz * g.dz);
+    g.idx = 1.0 / g.dx; g.idy = 1.0 / g.dy; g.idz = 1.0 / g.dz;
+    g.tdx = dt / g.dx; g.tdy = dt / g.dy; g.tdz = dt / g.dz;
+    g.idt = 1.0 / dt;
+    g.idx2 = 1.0 / (g.dx * g.dx); g.idy2 = 1.0 / (g.dy * g.dy); g.idz2 = 1.0 / (g.dz * g.dz);
     const size_t X = nx, Y = ny, Z = nz;
     const size_t cnt[9] = {(Z + 2) * (Y + 2) * (X + 3), (Z + 2) * (Y + 3) * (X + 2), (Z + 3) * (Y + 2) * (X + 2),
                            (Z + 2) * (Y + 2) * (X + 2), Z * Y * X, Z * Y * (X + 1), Z * (Y + 1) * X,
