@@ -146,11 +146,15 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int chunk = blk / tiles, tile = blk - chunk * tiles;
     const int x0 = (tile % gtx) * TX, y0 = (tile / gtx) * TY;
     const int z0 = chunk * chunk_gran * kZGran, z1 = min(nz, z0 + chunk_gran * kZGran);
-    const auto fp = as_global(f(p));
-    const auto up = as_global(u(p));
+    const long pl = (long)nx * ny;
+    // buffer resources over problem p's fields: a load's address is a lane
+    // byte offset (the column) plus a wave-uniform one (the plane), so the
+    // per-plane part is one scalar multiply-add instead of 64-bit address
+    // arithmetic per load
+    const BufRange<double> fb(wave_uniform(f(p)), (unsigned)(pl * nz * (long)sizeof(double)));
+    const BufRange<Tu> ub(wave_uniform(u(p)), (unsigned)(pl * nz * (long)sizeof(Tu)));
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int x = x0 + tx, y = y0 + ty;
-    const long pl = (long)nx * ny;
     const bool valid = x < nx && y < ny;
     // lanes outside the field read their nearest field point (lines their
     // neighbours read) and drop the value
@@ -184,7 +188,10 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(8))) voi
     // plane to plane (the loop is unrolled by D + 1), so no register holding a
     // load in flight is ever copied: a copy would wait for the load.
     const int zl = nz - 1;
-    auto zcl = [&](int zz) { return (long)(zz < 0 ? 0 : (zz > zl ? zl : zz)) * pl; };
+    const int plu = (int)(pl * (long)sizeof(Tu)), plf = (int)(pl * (long)sizeof(double));
+    auto zcl = [&](int zz) { return zz < 0 ? 0 : (zz > zl ? zl : zz); };
+    const int c0u = (int)(c0 * (long)sizeof(Tu)), c0f = (int)(c0 * (long)sizeof(double));
+    const int cHu = (int)(cH * (long)sizeof(Tu));
     auto uval = [&](Tu v, int zz) { return (valid && zz >= 0 && zz < nz) ? (double)v : 0.0; };
     struct Slot {
         Tu u;       // u at z+1 (centre column)
@@ -197,14 +204,14 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(8))) voi
     const bool halo_wave =
         sizeof(Tu) != 8 || __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * 64 < 2 * TX + 2 * TY;
     auto load_slot = [&](Slot& a, int zu, int zf) {
-        a.u = up[zcl(zu) + c0];
-        a.f = fp[zcl(zf) + c0];
-        if (halo_wave) a.h = up[zcl(zf) + cH];
+        a.u = ub.load(c0u, zcl(zu) * plu);
+        a.f = fb.load(c0f, zcl(zf) * plf);
+        if (halo_wave) a.h = ub.load(cHu, zcl(zf) * plu);
     };
     // prefetch depth in planes (three measured no faster than two for the
     // read-only FP64 pass: its memory parallelism is not what bounds it)
     constexpr int D = 2, NS = D + 1;
-    const Tu um0 = up[zcl(z0 - 1) + c0], uc0 = up[zcl(z0) + c0];
+    const Tu um0 = ub.load(c0u, zcl(z0 - 1) * plu), uc0 = ub.load(c0u, zcl(z0) * plu);
     Slot sl[NS];
     static_for<0, D>([&](auto kk) {
         constexpr int q = decltype(kk)::value;
