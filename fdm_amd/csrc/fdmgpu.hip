@@ -178,6 +178,7 @@ int work_reserve(const fdmgpu_plan_t* P, Work& W, int count) {
     if (W.cap >= count && W.r32) return FDMGPU_OK;
     const double* fcopy = W.fcopy;
     const size_t fcb = W.fcopy_bytes;
+    int* bad = W.bad;
     hipFree(W.r32);
     hipFree(W.partial);
     hipFree(W.done);
@@ -186,6 +187,7 @@ int work_reserve(const fdmgpu_plan_t* P, Work& W, int count) {
     W = Work{};
     W.fcopy = const_cast<double*>(fcopy);
     W.fcopy_bytes = fcb;
+    W.bad = bad;
     const long elems = plan_elems(P);
     const long nblk = partial_blocks(P);
     const size_t r32b = align256(sizeof(float) * elems * count);
@@ -603,6 +605,7 @@ int solve_refine_periodic(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, c
         pa.r32 = it < last ? W.r32 : nullptr;
         pa.check = true;
         pa.it = it;
+        pa.bad = it == last ? W.bad : nullptr;
         Timed tm(P, pa.r32 ? FDMGPU_K_RESIDUAL_RHS : FDMGPU_K_RESIDUAL, s);
         HIPCHK(launch_periodic_residual(pa));
     }
@@ -714,6 +717,7 @@ int solve_refine(fdmgpu_plan_t* P, Work& W, double* ans, Tab ans_tab, const doub
         ra.u = u1f && it == 1 ? (const double*)W.u32 : ans;
         ra.u_tab = u1f && it == 1 ? nullptr : ans_tab;
         ra.u_f32 = u1f && it == 1;
+        ra.bad = it == last ? W.bad : nullptr;
         Timed tm(P, ra.r32 ? FDMGPU_K_RESIDUAL_RHS : FDMGPU_K_RESIDUAL, s);
         HIPCHK(launch_residual(ra));
     }

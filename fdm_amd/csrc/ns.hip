@@ -8,7 +8,7 @@
 //   v  i in [0,nz+1], k in [-1,ny+1], j in [0,nx+1]
 //   w  i in [-1,nz+1], k in [0,ny+1], j in [0,nx+1]
 //   p  [0,n+1]^3;  x, RHS [1,n]^3;  F j in [0,nx];  G k in [0,ny];  H i in [0,nz]
-// A step is 6 kernels plus the Poisson solve on one stream; from the third
+// A step is 4 kernels plus the Poisson solve on one stream; from the third
 // step on it is replayed from a hipGraph captured from one step.
 // Arithmetic is FP64 except inside an FP32+refine Poisson solve.
 #include <hip/hip_runtime.h>
@@ -48,66 +48,71 @@ struct Fields {
 
 __device__ __forceinline__ double sq(double a) { return a * a; }
 
-// lid: u[nz+1][k][j] = 2 U0 - u[nz][k][j], k in [0, ny+1], j in [-1, nz+1]
-// (src/ns_cube.cpp:66-72; the x bound is nz+1 in the reference)
-__global__ void ns_lid_kernel(Grid g, double* __restrict__ u) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x - 1;
-    const int k = blockIdx.y;
-    if (j > g.nz + 1) return;
-    const auto U = as_global(u);
-    U[g.u(g.nz + 1, k, j)] = 2 * g.U0 - U[g.u(g.nz, k, j)];
-}
-
-// ghost layers (src/ns_cube.cpp:74-96): blockIdx.z = 0 u in x, 1 v in y, 2 w in z
-__global__ void ns_ghost_kernel(Grid g, double* __restrict__ u, double* __restrict__ v, double* __restrict__ w) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;   // fastest index of the face
+// init_bound (src/ns_cube.cpp:65-122) in one launch: the lid, the ghost
+// layers of u, v, w and the pressure on the faces.  The reference runs them
+// in sequence: lid u[nz+1][k][j] = 2 U0 - u[nz][k][j] (k in [0, ny+1],
+// j in [-1, nz+1]: the x bound is nz+1 in the reference), then the ghost
+// copies, then the pressure faces from the ghosts.  Here every location is
+// written by one thread with its final value, computed from values no thread
+// of the launch writes:
+//   task 0 (a = j, b = k): the lid row for j in [0, nz+1] other than the
+//     ghost columns j = -1, nx+1 (the ghost copies overwrite those after the
+//     lid in the reference);
+//   task 1 (a = k, b = i): u's ghost columns -- at i = nz+1 the copy of the
+//     lid-updated row, 2 U0 - u[nz][k][.], where the lid reached that column
+//     -- and the pressure's x faces (i in [1, nz], k in [1, ny]) with the
+//     ghost values as computed, not re-read;
+//   task 2 (a = j, b = i): v's ghost rows and the y faces;
+//   task 3 (a = j, b = k): w's ghost planes and the z faces.
+// The pressure faces read only interior pressure (indices 1..n), which no
+// thread writes.  The arithmetic is the reference's, so the results equal the
+// three sequential passes bitwise.
+__global__ void ns_bound_kernel(Grid g, double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+                                double* __restrict__ p) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
-    if (blockIdx.z == 0) {   // i = b in [0, nz+1], k = a in [0, ny+1]
-        if (a > g.ny + 1 || b > g.nz + 1) return;
-        const auto U = as_global(u);
-        U[g.u(b, a, -1)] = U[g.u(b, a, 1)];
-        U[g.u(b, a, g.nx + 1)] = U[g.u(b, a, g.nx - 1)];
-    } else if (blockIdx.z == 1) {   // i = b in [0, nz+1], j = a in [0, nx+1]
-        if (a > g.nx + 1 || b > g.nz + 1) return;
-        const auto V = as_global(v);
-        V[g.v(b, -1, a)] = V[g.v(b, 1, a)];
-        V[g.v(b, g.ny + 1, a)] = V[g.v(b, g.ny - 1, a)];
-    } else {   // k = b in [0, ny+1], j = a in [0, nx+1]
-        if (a > g.nx + 1 || b > g.ny + 1) return;
-        const auto W = as_global(w);
-        W[g.w(-1, b, a)] = W[g.w(1, b, a)];
-        W[g.w(g.nz + 1, b, a)] = W[g.w(g.nz - 1, b, a)];
-    }
-}
-
-// pressure on the faces (src/ns_cube.cpp:98-127): blockIdx.z = 0 x-faces
-// (i, k), 1 y-faces (i, j), 2 z-faces (k, j); face index b + 1, a + 1
-__global__ void ns_pface_kernel(Grid g, const double* __restrict__ u, const double* __restrict__ v,
-                                const double* __restrict__ w, double* __restrict__ p) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int b = blockIdx.y + 1;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const auto U = as_global(u);
+    const auto V = as_global(v);
+    const auto W = as_global(w);
     const auto P = as_global(p);
-    if (blockIdx.z == 0) {   // i = b, k = a
-        if (a > g.ny || b > g.nz) return;
-        const auto U = as_global(u);
-        const int i = b, k = a, nx = g.nx;
-        P[g.p(i, k, 0)] = P[g.p(i, k, 1)] - (U[g.u(i, k, 1)] - 2 * U[g.u(i, k, 0)] + U[g.u(i, k, -1)]) / g.Re / g.dx;
-        P[g.p(i, k, nx + 1)] =
-            P[g.p(i, k, nx)] - (U[g.u(i, k, nx + 1)] - 2 * U[g.u(i, k, nx)] + U[g.u(i, k, nx - 1)]) / g.Re / g.dx;
-    } else if (blockIdx.z == 1) {   // i = b, j = a
-        if (a > g.nx || b > g.nz) return;
-        const auto V = as_global(v);
-        const int i = b, j = a, ny = g.ny;
-        P[g.p(i, 0, j)] = P[g.p(i, 1, j)] - (V[g.v(i, 1, j)] - 2 * V[g.v(i, 0, j)] + V[g.v(i, -1, j)]) / g.Re / g.dy;
-        P[g.p(i, ny + 1, j)] =
-            P[g.p(i, ny, j)] - (V[g.v(i, ny + 1, j)] - 2 * V[g.v(i, ny, j)] + V[g.v(i, ny - 1, j)]) / g.Re / g.dy;
-    } else {   // k = b, j = a
-        if (a > g.nx || b > g.ny) return;
-        const auto W = as_global(w);
-        const int k = b, j = a, nz = g.nz;
-        P[g.p(0, k, j)] = P[g.p(1, k, j)] - (W[g.w(1, k, j)] - 2 * W[g.w(0, k, j)] + W[g.w(-1, k, j)]) / g.Re / g.dz;
-        P[g.p(nz + 1, k, j)] =
-            P[g.p(nz, k, j)] - (W[g.w(nz + 1, k, j)] - 2 * W[g.w(nz, k, j)] + W[g.w(nz - 1, k, j)]) / g.Re / g.dz;
+    if (blockIdx.z == 0) {   // lid: j = a - 1 in [-1, nz+1], k = b in [0, ny+1]
+        const int j = a - 1, k = b;
+        if (j > nz + 1 || k > ny + 1 || j == -1 || j == nx + 1) return;
+        U[g.u(nz + 1, k, j)] = 2 * g.U0 - U[g.u(nz, k, j)];
+    } else if (blockIdx.z == 1) {   // k = a in [0, ny+1], i = b in [0, nz+1]
+        const int k = a, i = b;
+        if (k > ny + 1 || i > nz + 1) return;
+        const bool lid = i == nz + 1;
+        const double gm = lid ? 2 * g.U0 - U[g.u(nz, k, 1)] : U[g.u(i, k, 1)];
+        const double gp = lid ? (nx - 1 <= nz + 1 ? 2 * g.U0 - U[g.u(nz, k, nx - 1)] : U[g.u(nz + 1, k, nx - 1)])
+                              : U[g.u(i, k, nx - 1)];
+        U[g.u(i, k, -1)] = gm;
+        U[g.u(i, k, nx + 1)] = gp;
+        if (i >= 1 && i <= nz && k >= 1 && k <= ny) {
+            P[g.p(i, k, 0)] = P[g.p(i, k, 1)] - (U[g.u(i, k, 1)] - 2 * U[g.u(i, k, 0)] + gm) / g.Re / g.dx;
+            P[g.p(i, k, nx + 1)] = P[g.p(i, k, nx)] - (gp - 2 * U[g.u(i, k, nx)] + U[g.u(i, k, nx - 1)]) / g.Re / g.dx;
+        }
+    } else if (blockIdx.z == 2) {   // j = a in [0, nx+1], i = b in [0, nz+1]
+        const int j = a, i = b;
+        if (j > nx + 1 || i > nz + 1) return;
+        const double gm = V[g.v(i, 1, j)], gp = V[g.v(i, ny - 1, j)];
+        V[g.v(i, -1, j)] = gm;
+        V[g.v(i, ny + 1, j)] = gp;
+        if (i >= 1 && i <= nz && j >= 1 && j <= nx) {
+            P[g.p(i, 0, j)] = P[g.p(i, 1, j)] - (V[g.v(i, 1, j)] - 2 * V[g.v(i, 0, j)] + gm) / g.Re / g.dy;
+            P[g.p(i, ny + 1, j)] = P[g.p(i, ny, j)] - (gp - 2 * V[g.v(i, ny, j)] + V[g.v(i, ny - 1, j)]) / g.Re / g.dy;
+        }
+    } else {   // j = a in [0, nx+1], k = b in [0, ny+1]
+        const int j = a, k = b;
+        if (j > nx + 1 || k > ny + 1) return;
+        const double gm = W[g.w(1, k, j)], gp = W[g.w(nz - 1, k, j)];
+        W[g.w(-1, k, j)] = gm;
+        W[g.w(nz + 1, k, j)] = gp;
+        if (k >= 1 && k <= ny && j >= 1 && j <= nx) {
+            P[g.p(0, k, j)] = P[g.p(1, k, j)] - (W[g.w(1, k, j)] - 2 * W[g.w(0, k, j)] + gm) / g.Re / g.dz;
+            P[g.p(nz + 1, k, j)] = P[g.p(nz, k, j)] - (gp - 2 * W[g.w(nz, k, j)] + W[g.w(nz - 1, k, j)]) / g.Re / g.dz;
+        }
     }
 }
 
@@ -248,11 +253,6 @@ __global__ __launch_bounds__(kTileX * kTileY) void ns_update_kernel(Grid g, Fiel
     as_global(f.p)[g.p(i, k, j)] = xc;
 }
 
-// Marks the run when the pressure solve's final relative residual (the
-// refinement's per-problem result) is above the tolerance or not a number.
-__global__ void ns_converged_kernel(const double* __restrict__ resid, const double tol, int* __restrict__ bad) {
-    if (threadIdx.x == 0 && !(resid[0] <= tol)) bad[0] = 1;
-}
 
 // Largest |value| of each of the nine arrays, ghost layers included, as
 // tensor::maxabs (src/tensor.h:269-273): the running maximum starts at 0 and
@@ -337,15 +337,12 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s, hipEvent_t* ev = nullptr) {
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], s) : hipSuccess; };
     (void)hipGetLastError();
     NS_HIPCHK(mark(0));
-    hipLaunchKernelGGL(ns_lid_kernel, dim3((g.nz + 3 + 63) / 64, g.ny + 2), dim3(64), 0, s, g, N->f.u);
-    NS_HIPCHK(hipGetLastError());
-    const int fa = std::max(g.nx, g.ny) + 2, fb = std::max(g.nz, g.ny) + 2;
-    hipLaunchKernelGGL(ns_ghost_kernel, dim3((fa + 63) / 64, fb, 3), dim3(64), 0, s, g, N->f.u, N->f.v, N->f.w);
-    NS_HIPCHK(hipGetLastError());
-    const int pa = std::max(g.nx, g.ny), pb = std::max(g.nz, g.ny);
-    hipLaunchKernelGGL(ns_pface_kernel, dim3((pa + 63) / 64, pb, 3), dim3(64), 0, s, g, N->f.u, N->f.v, N->f.w,
-                       N->f.p);
-    NS_HIPCHK(hipGetLastError());
+    {
+        const int fa = std::max(std::max(g.nz + 3, g.ny + 2), g.nx + 2), fb = std::max(g.nz, g.ny) + 2;
+        hipLaunchKernelGGL(ns_bound_kernel, dim3((fa + 63) / 64, fb, 4), dim3(64), 0, s, g, N->f.u, N->f.v, N->f.w,
+                           N->f.p);
+        NS_HIPCHK(hipGetLastError());
+    }
     NS_HIPCHK(mark(1));
     const TileGrid tf = tile_grid(g.nx + 1, g.ny + 1, g.nz + 1), ti = tile_grid(g.nx, g.ny, g.nz);
     hipLaunchKernelGGL(ns_fgh_kernel, dim3(tf.blocks()), b2, 0, s, g, N->f, tf);
@@ -361,15 +358,15 @@ int ns_enqueue_step(fdmgpu_ns_t* N, hipStream_t s, hipEvent_t* ev = nullptr) {
     // those of a cold solve.
     const bool refine = N->plan->precision == FDMGPU_F32_REFINE64;
     const bool warm = refine && N->warm_ok;
+    // a projection that misses the tolerance marks the run: the solve's final
+    // norm check sets d_bad on the device (no host round trip, no extra
+    // launch); only this step's solves carry the flag
+    N->plan->w.bad = N->d_bad;
     int rc = warm ? plan_solve_iters(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s, 2,
                                      std::max(2, N->plan->max_iter - 1))
                   : plan_solve(N->plan, N->plan->w, N->f.x, nullptr, N->f.RHS, nullptr, 1, s);
+    N->plan->w.bad = nullptr;
     if (rc) return rc;
-    if (refine) {   // a projection that missed the tolerance marks the run (no host round trip)
-        hipLaunchKernelGGL(ns_converged_kernel, dim3(1), dim3(64), 0, s, (const double*)N->plan->w.resid,
-                           N->plan->rtol, N->d_bad);
-        NS_HIPCHK(hipGetLastError());
-    }
     NS_HIPCHK(mark(3));
     hipLaunchKernelGGL(ns_update_kernel, dim3(ti.blocks()), b2, 0, s, g, N->f, ti);
     NS_HIPCHK(hipGetLastError());

@@ -25,6 +25,7 @@ struct Work {
     int cap = 0;
     double* fcopy = nullptr;            // f kept when a refine solve runs in place
     size_t fcopy_bytes = 0;
+    int* bad = nullptr;                 // device flag set when a solve ends above tolerance (or null)
 };
 
 using Tab = const void* const*;

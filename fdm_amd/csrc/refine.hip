@@ -303,10 +303,12 @@ static void launch_residual_tile(const ResidualArgs& a, const BatchPtr<const dou
 constexpr int kCheckBlock = 1024;
 constexpr int kCheckUnroll = 8;
 
+// bad (the final certification of a solve, or null): set to 1 when a
+// problem checked here ends above the tolerance (or its residual is NaN).
 __global__ __launch_bounds__(kCheckBlock) void check_kernel(const double* __restrict__ partial, const int nblk,
                                                            const double tol, int* __restrict__ done,
                                                            int* __restrict__ iters, double* __restrict__ resid,
-                                                           const int it) {
+                                                           const int it, int* __restrict__ bad) {
     __shared__ double sh[2 * (kCheckBlock / 64)];
     const int p = blockIdx.x;
     if (done[p]) return;
@@ -349,6 +351,7 @@ __global__ __launch_bounds__(kCheckBlock) void check_kernel(const double* __rest
         resid[p] = rel;
         iters[p] = it;
         if (rel <= tol) done[p] = 1;
+        else if (bad) bad[0] = 1;
     }
 }
 
@@ -390,13 +393,14 @@ hipError_t launch_residual(const ResidualArgs& a) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(check_kernel, dim3(a.batch), dim3(kCheckBlock), 0, a.stream, a.partial, (int)nblk, a.tol, a.done,
-                       a.iters, a.resid, a.it);
+                       a.iters, a.resid, a.it, a.bad);
     return hipGetLastError();
 }
 
 hipError_t launch_check(const double* partial, int nblk, double tol, int* done, int* iters, double* resid, int it,
                         int batch, hipStream_t s) {
-    hipLaunchKernelGGL(check_kernel, dim3(batch), dim3(kCheckBlock), 0, s, partial, nblk, tol, done, iters, resid, it);
+    hipLaunchKernelGGL(check_kernel, dim3(batch), dim3(kCheckBlock), 0, s, partial, nblk, tol, done, iters, resid, it,
+                       (int*)nullptr);
     return hipGetLastError();
 }
 
@@ -559,7 +563,7 @@ hipError_t launch_periodic_residual(const PeriodicResidualArgs& a) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !a.check) return e;
     hipLaunchKernelGGL(check_kernel, dim3(a.batch), dim3(kCheckBlock), 0, a.stream, (const double*)a.partial, a.nblk,
-                       a.tol, a.done, a.iters, a.resid, a.it);
+                       a.tol, a.done, a.iters, a.resid, a.it, a.bad);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ struct ResidualArgs {
     const void* const* f_tab;   // per-problem pointers (device) or null
     const void* const* u_tab;
     bool u_f32 = false;         // u holds FP32 values (3-D only)
+    int* bad = nullptr;         // final certification: set to 1 when a problem ends above tol
 };
 
 hipError_t launch_residual(const ResidualArgs& a);
@@ -62,6 +63,7 @@ struct PeriodicResidualArgs {
     double* resid;
     int it;
     hipStream_t stream;
+    int* bad = nullptr;          // final certification: set to 1 when a problem ends above tol
 };
 hipError_t launch_periodic_fmean(const PeriodicResidualArgs& a);
 hipError_t launch_periodic_residual(const PeriodicResidualArgs& a);
