@@ -800,9 +800,11 @@ def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 64
         t_end = time.perf_counter()
         run.submit_s = t_subs - t_start
         lat, resid, ok = [], [], True
-        for tk, delay in tick:
+        run.large = []
+        for (tk, delay), shp in zip(tick, reqs):
             r = S.poll(tk)
             lat.append(r.latency_ms + 1e3 * delay)
+            run.large.append(shp[1:] == (255, 255, 255))
             resid.append(r.rel_residual)
             ok = ok and r.status == 0
             S.forget(tk)
@@ -828,9 +830,14 @@ def leg_c4(duration_s: float = 2.0, budget_gb: float = 24.0, max_batch: int = 64
         reqs = uneven_mix(n, seed + tag, big_frac=0.05)
         gaps = rng.exponential(1.0 / rate, n)
         dt, lat, resid, ok = run(reqs, gaps)
+        small = [x for x, big in zip(lat, run.large) if not big]
+        large = [x for x, big in zip(lat, run.large) if big]
         out["rates"].append({"offered_per_s": round(rate, 1), "requests": n,
                              "achieved_solves_per_s": round(n / dt, 1),
                              "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
+                             "p99_ms_excluding_255": round(pct(small, 99), 3) if small else None,
+                             "p50_ms_255": round(pct(large, 50), 3) if large else None,
+                             "p99_ms_255": round(pct(large, 99), 3) if large else None,
                              "max_rel_residual": max(resid), "all_converged": ok})
     S.close()
     # FP64 one request at a time (no batching, no overlap), same mix
