@@ -27,7 +27,20 @@ struct BatchPtr {
     __host__ __device__ T* operator()(int p) const {
         return tab ? (T*)tab[p] : base + (long)p * bs;
     }
+    // The same for a wave-uniform p: the table entry comes by a scalar load,
+    // counted by lgkmcnt.  A vector load would be counted by vmcnt, and
+    // waiting for it would also wait for every vector load and store the
+    // wave issued before it (the counter retires in order).
+    __device__ T* uniform(int p) const {
+        if (!tab) return base + (long)p * bs;
+        using CP = const __attribute__((address_space(4))) unsigned long long*;
+        return (T*)((CP)tab)[p];
+    }
 };
+
+// Waits until every vector memory access the wave has issued is complete
+// (s_waitcnt vmcnt(0); expcnt and lgkmcnt left alone).
+__device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // A pointer every lane of the wave holds (e.g. read from a per-problem table
 // at a wave-uniform index), moved to scalar registers: addresses derived from

@@ -237,20 +237,32 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
     auto load_plane = [&](const int qq) {
         int p, z;
         split(qq, p, z);
-        const BufRange<double> src(wave_uniform(in(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
+        const BufRange<double> src(wave_uniform(in.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
         if (x0 < n) {
 #pragma unroll
             for (int i = 0; i < ITER; ++i)
                 if (R * i + R <= n || r0 + R * i < n) v[i] = src.load(lane_in, (int)(R * i * n * sizeof(double)));
         }
     };
-    load_plane(q);
-    for (;;) {
+    auto stage = [&] {
         if (x0 < n) {
 #pragma unroll
             for (int i = 0; i < ITER; ++i)
                 if (R * i + R <= n || r0 + R * i < n) lds[slot0 + R * i * P] = v[i];
         }
+    };
+    // Staging sits at the end of the loop body, behind the store phase, and
+    // the first plane is staged before the loop, after which every load so
+    // far has completed: nothing the loop reads from global memory is then
+    // pending at the loop head.  (vmcnt retires in order; a value pending
+    // there, such as the lane constants loaded above, gets its wait inside
+    // the loop, where it also waits for the whole prefetch in flight.)
+    // Within an iteration the prefetch is issued before the store phase's
+    // stores, so staging waits for the prefetch only, never for those stores.
+    load_plane(q);
+    stage();
+    wait_vmem_all();
+    for (;;) {
         __syncthreads();
         const int qn = q + (int)gridDim.x;
         if (qn < total) load_plane(qn);   // in flight until the next staging
@@ -274,12 +286,27 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
             asm volatile("" : "+v"(t3));
             int p, z;
             split(q, p, z);
-            const auto dst = as_global(wave_uniform(out(p)) + (long)z * plane_pts);
-            plane_sweep<LOGN, NT, n>(t3, [&](int, int y, int x) { dst[y * n + x] = lds[LX::at(IO::out(y), IO::out(x))]; });
+            const BufRange<double> dst(wave_uniform(out.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
+            // Every lane stores every row, with no branch: a lane outside the
+            // plane (x or y >= n) reads a slot inside it and stores beyond
+            // the buffer's range, which the hardware drops.  (A branch around
+            // the stores would make the compiler assume at staging that they
+            // might not have been issued, and count the waits for the
+            // prefetch without them: staging would then wait for the stores.)
+            const int xo = t3 & (N - 1), yo = t3 >> LOGN;
+            const int xs = xo < n ? xo : n - 1;
+            const int lane_out = xo < n ? (int)((yo * n + xo) * sizeof(double)) : (int)(plane_pts * sizeof(double));
+#pragma unroll
+            for (int i = 0; i < ITER; ++i) {
+                const int y = yo + R * i;
+                const int ys = (R * i + R <= n || y < n) ? y : n - 1;
+                dst.store(lds[LX::at(IO::out(ys), IO::out(xs))], lane_out, (int)(R * i * n * sizeof(double)));
+            }
         }
         if (qn >= total) break;
         q = qn;
         __syncthreads();   // the store phase has read the plane before the next one is staged
+        stage();
     }
 }
 
