@@ -210,9 +210,15 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* lds = reinterpret_cast<double*>(smem_raw);
     constexpr unsigned plane_pts = (unsigned)n * n;
-    const int x0 = tid & (N - 1), r0 = tid >> LOGN;
-    const int slot0 = opaque(LX::at(r0 + o, x0 + o));   // element i: slot0 + R i P
-    const int lane_in = (int)((r0 * n + x0) * sizeof(double));
+    // Thread (r0, x0) loads, stages and stores column x0 of rows ITER r0 + i,
+    // i < ITER: a row group r0 is two whole waves, so r0 is wave-uniform and
+    // every row-dependent offset is a compile-time constant within a group.
+    // Lanes outside the plane (x0 >= n, or the rows >= n of the last group)
+    // load in bounds or past the buffer's range (which reads 0), stage into
+    // the pad column, and store past the range (which is dropped).
+    const int x0 = tid & (N - 1), r0 = __builtin_amdgcn_readfirstlane(tid >> LOGN);
+    const int slot0 = opaque(LX::at(ITER * r0 + o, x0 + o));   // element i: slot0 + i P
+    const int lane_in = (int)((ITER * r0 * n + x0) * sizeof(double));
 
     // the DST's shifted sines come from an LDS copy behind the plane (the
     // prefetch takes their registers)
@@ -238,18 +244,13 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
         int p, z;
         split(qq, p, z);
         const BufRange<double> src(wave_uniform(in.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
-        if (x0 < n) {
 #pragma unroll
-            for (int i = 0; i < ITER; ++i)
-                if (R * i + R <= n || r0 + R * i < n) v[i] = src.load(lane_in, (int)(R * i * n * sizeof(double)));
-        }
+        for (int i = 0; i < ITER; ++i) v[i] = src.load(lane_in, (int)(i * n * sizeof(double)));
     };
     auto stage = [&] {
-        if (x0 < n) {
 #pragma unroll
-            for (int i = 0; i < ITER; ++i)
-                if (R * i + R <= n || r0 + R * i < n) lds[slot0 + R * i * P] = v[i];
-        }
+        for (int i = 0; i < ITER; ++i)   // a row >= n would land on the sines behind the plane
+            if (ITER * (R - 1) + i < n || ITER * r0 + i < n) lds[slot0 + i * P] = v[i];
     };
     // Staging sits at the end of the loop body, behind the store phase, and
     // the first plane is staged before the loop, after which every load so
@@ -293,14 +294,15 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
             // the stores would make the compiler assume at staging that they
             // might not have been issued, and count the waits for the
             // prefetch without them: staging would then wait for the stores.)
-            const int xo = t3 & (N - 1), yo = t3 >> LOGN;
-            const int xs = xo < n ? xo : n - 1;
-            const int lane_out = xo < n ? (int)((yo * n + xo) * sizeof(double)) : (int)(plane_pts * sizeof(double));
+            const int xo = t3 & (N - 1);
+            const int xs = opaque(IO::out(xo < n ? xo : n - 1));
+            const int lane_out = xo < n ? (int)((ITER * r0 * n + xo) * sizeof(double)) : (int)(plane_pts * sizeof(double));
+            // the row part of each LDS slot is wave-uniform (scalar
+            // arithmetic); a lane adds its column slot
 #pragma unroll
             for (int i = 0; i < ITER; ++i) {
-                const int y = yo + R * i;
-                const int ys = (R * i + R <= n || y < n) ? y : n - 1;
-                dst.store(lds[LX::at(IO::out(ys), IO::out(xs))], lane_out, (int)(R * i * n * sizeof(double)));
+                const int y = ITER * r0 + i, ys = y < n - 1 ? y : n - 1;
+                dst.store(lds[LX::at(IO::out(ys), 0) + xs], lane_out, (int)(i * n * sizeof(double)));
             }
         }
         if (qn >= total) break;
