@@ -468,7 +468,7 @@ struct fdmgpu_server_t {
     double rtol = 1e-10;          // refinement settings of the server's plans
     int max_iter = 4;
     std::vector<Batch*> running, spare;
-    std::thread worker;
+    std::vector<std::thread> workers;
     bool stop = false;
     int worker_error = FDMGPU_OK;
     unsigned long long next_ticket = 1;
@@ -526,6 +526,16 @@ int launch_batch(fdmgpu_server_t* S, Batch* b) {
     SRV_HIPCHK(hipEventRecord(b->ev, s));
     return FDMGPU_OK;
 }
+
+// Worker threads.  Admission, batching and completion run under the server
+// lock; a batch's launch (a dozen or so kernel launches and copies) runs
+// outside it, so several workers can issue batches to their streams at once.
+// Two measured the same as one on the mixed serving stream (the host's
+// issue rate is not what bounds it), so one it is.
+constexpr int kWorkers = 1;
+
+// Compute units the large-problem stream leaves to the others (0: none).
+constexpr int kReservedCUs = 32;
 
 // Problems of at least this many points are "large" (a 255^3 problem has
 // 16.6M; 127^3 has 2.0M): they get stream 0 to themselves.
@@ -728,7 +738,26 @@ int fdmgpu_server_create(fdmgpu_server* server, unsigned long long hbm_budget, i
     if (hipGetDevice(&S->device) != hipSuccess) rc = FDMGPU_ERR_DEVICE;
     for (int i = 0; i < (nstreams ? nstreams : 4) && !rc; i++) {
         hipStream_t s;
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) rc = FDMGPU_ERR_DEVICE;
+        hipError_t e;
+        int cus = 0;
+        if (i == 0 && nstreams != 1 && kReservedCUs > 0 &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, S->device) == hipSuccess &&
+            cus > 2 * kReservedCUs) {
+            // the large-problem stream leaves kReservedCUs compute units to the
+            // other streams, so a small request does not wait for a large
+            // solve's blocks to drain.  CU c is reserved when
+            // (c mod 32) mod q == (c / 32) mod q, q = cus / kReservedCUs: the
+            // same number in every 32 consecutive CUs and, on 256 CUs, in
+            // every XCD whether CUs are numbered per XCD or round-robin
+            const int q = std::max(1, cus / kReservedCUs);
+            std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+            for (int c = 0; c < cus; c++)
+                if ((c % 32) % q != (c / 32) % q) mask[c / 32] |= 1u << (c % 32);
+            e = hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data());
+        } else {
+            e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        }
+        if (e != hipSuccess) rc = FDMGPU_ERR_DEVICE;
         else S->streams.push_back(s);
     }
     S->stream_load.assign(S->streams.size(), 0);
@@ -739,7 +768,7 @@ int fdmgpu_server_create(fdmgpu_server* server, unsigned long long hbm_budget, i
         return rc;
     }
     S->arena.init(S->d_arena, hbm_budget);
-    S->worker = std::thread(worker_main, S);
+    for (int i = 0; i < kWorkers; i++) S->workers.emplace_back(worker_main, S);
     *server = S;
     return FDMGPU_OK;
 }
@@ -832,7 +861,8 @@ int fdmgpu_server_destroy(fdmgpu_server S) {
         S->stop = true;
     }
     S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
+    for (std::thread& w : S->workers)
+        if (w.joinable()) w.join();
     (void)hipSetDevice(S->device);
     for (hipStream_t s : S->streams) (void)hipStreamSynchronize(s);
     for (Batch* b : S->spare) free_batch(b);

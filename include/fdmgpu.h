@@ -266,7 +266,9 @@ int fdmgpu_group_destroy(fdmgpu_group group);
  * asynchronously.  Each stream holds at most two batches in flight: under
  * load, arrivals wait on the host and join larger batches.  Problems of at
  * least 2^22 points run on the first stream, smaller ones on the others
- * (nstreams > 1).  A request that must wait (no stream of its size class
+ * (nstreams > 1); the first stream then runs on all but 32 compute units
+ * (a CU mask; it synchronizes with the legacy null stream), which stay free
+ * for the small requests' kernels.  A request that must wait (no stream of its size class
  * free, or its scratch does not fit yet) can be passed by later requests of
  * other solvers; while it waits for scratch, at most 8 batches pass it before
  * admission waits for it.  Solver tables stay resident between requests.  A
