@@ -549,7 +549,10 @@ def measured_traffic(kernel: str, n: int, batch: int):
     return None if k is None else k.get("traffic_bytes_per_launch")
 
 
-def run_gpu(mode: str, n: int, batch: int, steps: int, warmup: int, f_host: np.ndarray):
+def run_gpu(mode: str, n: int, batch: int, steps: int, warmup: int, f_host: np.ndarray, instrument: bool = True):
+    """Times `steps` batched solves.  instrument: events around the dominant
+    kernel's launches inside the timed region (the roofline figure of the
+    headline path); without it the timed region runs uninstrumented."""
     import fdm_amd
     from fdm_amd import hip
 
@@ -579,7 +582,10 @@ def run_gpu(mode: str, n: int, batch: int, steps: int, warmup: int, f_host: np.n
     # timed region: events around the dominant kernel's launches only (its
     # average launch time is the roofline figure; the other launches run
     # uninstrumented)
-    solver.set_timing(True, kernels=[dom])
+    if instrument:
+        solver.set_timing(True, kernels=[dom])
+    else:
+        solver.set_timing(False)
     ev0, ev1 = hip.Event(), hip.Event()
     hip.synchronize()
     t0 = time.perf_counter()
@@ -1115,7 +1121,7 @@ def main():
         return
     progress(f"head: {B} x {n}^3 FP32+refine and FP64")
     res_r = run_gpu("refine", n, B, args.steps, args.warmup, F)
-    res_d = run_gpu("f64", n, B, args.steps, args.warmup, F)
+    res_d = run_gpu("f64", n, B, args.steps, args.warmup, F, instrument=False)
     fcpu = F[0].copy()
 
     solves = B * args.steps
