@@ -136,6 +136,14 @@ inline TileGrid tile_grid(int nx, int ny, int nz) {
     return TileGrid{(nx + kTileX - 1) / kTileX, (ny + kTileY - 1) / kTileY, nz};
 }
 
+// Row of the tile a thread works on: a wavefront is one row of 64 threads
+// (blockDim.x == kTileX), so the row is wave-uniform; said so, every row and
+// plane part of an address is scalar arithmetic and a lane adds only its x.
+static_assert(kTileX == 64, "a tile row is one wavefront");
+__device__ __forceinline__ int tile_row(const int by) {
+    return __builtin_amdgcn_readfirstlane(by * kTileY + (int)threadIdx.y);
+}
+
 __device__ __forceinline__ bool xcd_tile(const TileGrid tg, int& bx, int& by, int& bz) {
     const int per = (int)gridDim.x / kXcds;
     const int T = ((int)blockIdx.x % kXcds) * per + (int)blockIdx.x / kXcds;
@@ -153,7 +161,7 @@ __global__ __launch_bounds__(kTileX * kTileY) void ns_fgh_kernel(Grid g, Fields 
     int bx, by, bz;
     if (!xcd_tile(tg, bx, by, bz)) return;
     const int j = bx * kTileX + (int)threadIdx.x;
-    const int k = by * kTileY + (int)threadIdx.y;
+    const int k = tile_row(by);
     const int i = bz;
     if (j > g.nx || k > g.ny) return;
     const auto U = as_global((const double*)f.u);
@@ -208,7 +216,7 @@ __global__ __launch_bounds__(kTileX * kTileY) void ns_rhs_kernel(Grid g, Fields 
     int bx, by, bz;
     if (!xcd_tile(tg, bx, by, bz)) return;
     const int j = bx * kTileX + (int)threadIdx.x + 1;
-    const int k = by * kTileY + (int)threadIdx.y + 1;
+    const int k = tile_row(by) + 1;
     const int i = bz + 1;
     if (j > g.nx || k > g.ny) return;
     const auto Fa = as_global((const double*)f.F);
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(kTileX * kTileY) void ns_update_kernel(Grid g, Fiel
     int bx, by, bz;
     if (!xcd_tile(tg, bx, by, bz)) return;
     const int j = bx * kTileX + (int)threadIdx.x + 1;
-    const int k = by * kTileY + (int)threadIdx.y + 1;
+    const int k = tile_row(by) + 1;
     const int i = bz + 1;
     if (j > g.nx || k > g.ny) return;
     const auto X = as_global((const double*)f.x);
