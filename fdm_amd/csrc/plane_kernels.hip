@@ -180,15 +180,23 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
 // block fills a CU; the one-plane-per-block kernel then leaves HBM idle while
 // the block transforms and the CU idle while it loads.  Here a block walks
 // planes q = blockIdx.x, blockIdx.x + gridDim.x, ... (about the same count per
-// block: the grid is sized so), and right after plane q is staged into LDS it
-// issues the global loads of its next plane into registers; they are in
-// flight during the x and y passes and the store phase of plane q.  The plane
-// sits in LDS in the b64-conflict-free PlaneLayoutB64 (pitch kStreamPitch,
-// jobs dealt to pencils out of order); each pencil's transform and the order
-// of every floating-point operation in it are those of plane_dst_kernel, so
-// results are bitwise the same.
+// block: the grid is sized so), and while plane q is transformed the global
+// loads of its next plane are in flight in registers: the first kStreamSplit
+// rows of a thread's share issued before the x pass, the rest before the y
+// pass.  After the y pass the outputs are read into registers and, once every
+// wave has read its share, stored kStreamChunk rows at a time between the
+// next plane's staging writes.  (A wave issues its memory instructions in
+// order and stalls while the memory pipeline is full: a whole plane of loads
+// or of stores issued at once held every wave for about 4k cycles per plane,
+// measured with s_memtime stamps.)  The plane sits in LDS in the
+// b64-conflict-free PlaneLayoutB64 (pitch kStreamPitch, jobs dealt to pencils
+// out of order); each pencil's transform and the order of every
+// floating-point operation in it are those of plane_dst_kernel, so results
+// are bitwise the same.
 // ---------------------------------------------------------------------------
 constexpr int kStreamPitch = 130;
+constexpr int kStreamSplit = 16;   // rows (of 32 per thread) prefetched before the x pass
+constexpr int kStreamChunk = 8;    // rows stored between staging writes
 
 template <int KIND>
 __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu(2))) void plane_stream_kernel(
@@ -240,18 +248,26 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
         p = __builtin_amdgcn_readfirstlane(qq / nplanes);
         z = __builtin_amdgcn_readfirstlane(qq - p * nplanes);
     };
-    auto load_plane = [&](const int qq) {
+    // rows [LO, HI) of the thread's share of plane qq into v
+    auto load_rows = [&](const int qq, auto loc, auto hic) {
+        constexpr int LO = decltype(loc)::value, HI = decltype(hic)::value;
         int p, z;
         split(qq, p, z);
         const BufRange<double> src(wave_uniform(in.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
 #pragma unroll
-        for (int i = 0; i < ITER; ++i) v[i] = src.load(lane_in, (int)(i * n * sizeof(double)));
+        for (int i = LO; i < HI; ++i) v[i] = src.load(lane_in, (int)(i * n * sizeof(double)));
     };
-    auto stage = [&] {
+    using R0 = std::integral_constant<int, 0>;
+    using RH = std::integral_constant<int, kStreamSplit>;
+    using R1 = std::integral_constant<int, ITER>;
+    auto load_plane = [&](const int qq) { load_rows(qq, R0{}, R1{}); };
+    auto stage_rows = [&](auto loc, auto hic) {
+        constexpr int LO = decltype(loc)::value, HI = decltype(hic)::value;
 #pragma unroll
-        for (int i = 0; i < ITER; ++i)   // a row >= n would land on the sines behind the plane
+        for (int i = LO; i < HI; ++i)   // a row >= n would land on the sines behind the plane
             if (ITER * (R - 1) + i < n || ITER * r0 + i < n) lds[slot0 + i * P] = v[i];
     };
+    auto stage = [&] { stage_rows(std::integral_constant<int, 0>{}, std::integral_constant<int, ITER>{}); };
     // Staging sits at the end of the loop body, behind the store phase, and
     // the first plane is staged before the loop, after which every load so
     // far has completed: nothing the loop reads from global memory is then
@@ -266,7 +282,10 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
     for (;;) {
         __syncthreads();
         const int qn = q + (int)gridDim.x;
-        if (qn < total) load_plane(qn);   // in flight until the next staging
+        // the next plane's first rows, in flight until the next staging; the
+        // rest are issued after the x pass (issuing a whole plane at once
+        // stalls the wave while the memory pipeline takes them)
+        if (qn < total) load_rows(qn, R0{}, RH{});
         __builtin_amdgcn_sched_barrier(0);
         // the passes' addresses are recomputed per plane from a thread index
         // the compiler cannot hoist out of the loop: kept live across the
@@ -275,6 +294,9 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
         asm volatile("" : "+v"(t1));
         pencil_pairs<KIND, double, LOGN, N / 2, LX, NT, Lane>(lds, L, sx, t1);   // pencils = rows    (x)
         __syncthreads();
+        if constexpr (kStreamSplit < ITER)
+            if (qn < total) load_rows(qn, RH{}, R1{});
+        __builtin_amdgcn_sched_barrier(0);
         int t2 = tid;
         asm volatile("" : "+v"(t2));
         if constexpr (KIND == kDst)
@@ -282,33 +304,52 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
         else
             pencil_pairs<KIND, double, LOGN, N / 2, LY, NT, Lane>(lds, L, sy, t2);
         __syncthreads();
-        {
-            int t3 = tid;
-            asm volatile("" : "+v"(t3));
-            int p, z;
-            split(q, p, z);
-            const BufRange<double> dst(wave_uniform(out.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
-            // Every lane stores every row, with no branch: a lane outside the
-            // plane (x or y >= n) reads a slot inside it and stores beyond
-            // the buffer's range, which the hardware drops.  (A branch around
-            // the stores would make the compiler assume at staging that they
-            // might not have been issued, and count the waits for the
-            // prefetch without them: staging would then wait for the stores.)
-            const int xo = t3 & (N - 1);
-            const int xs = opaque(IO::out(xo < n ? xo : n - 1));
-            const int lane_out = xo < n ? (int)((ITER * r0 * n + xo) * sizeof(double)) : (int)(plane_pts * sizeof(double));
-            // the row part of each LDS slot is wave-uniform (scalar
-            // arithmetic); a lane adds its column slot
+        // store phase: the plane's outputs into registers, then, once every
+        // wave has read its share, their global stores interleaved with the
+        // next plane's staging (a plane of stores issued at once stalls the
+        // wave while the memory pipeline takes them; staged in between, the
+        // LDS writes proceed meanwhile)
+        int t3 = tid;
+        asm volatile("" : "+v"(t3));
+        int p, z;
+        split(q, p, z);
+        const BufRange<double> dst(wave_uniform(out.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(double));
+        // Every lane stores every row, with no branch: a lane outside the
+        // plane (x or y >= n) reads a slot inside it and stores beyond the
+        // buffer's range, which the hardware drops.  (A branch around the
+        // stores would make the compiler assume at staging that they might
+        // not have been issued, and count the waits for the prefetch without
+        // them: staging would then wait for the stores.)
+        const int xo = t3 & (N - 1);
+        const int xs = opaque(IO::out(xo < n ? xo : n - 1));
+        const int lane_out = xo < n ? (int)((ITER * r0 * n + xo) * sizeof(double)) : (int)(plane_pts * sizeof(double));
+        double o[ITER];
+        // the row part of each LDS slot is wave-uniform (scalar arithmetic);
+        // a lane adds its column slot
 #pragma unroll
-            for (int i = 0; i < ITER; ++i) {
-                const int y = ITER * r0 + i, ys = y < n - 1 ? y : n - 1;
-                dst.store(lds[LX::at(IO::out(ys), 0) + xs], lane_out, (int)(i * n * sizeof(double)));
-            }
+        for (int i = 0; i < ITER; ++i) {
+            const int y = ITER * r0 + i, ys = y < n - 1 ? y : n - 1;
+            o[i] = lds[LX::at(IO::out(ys), 0) + xs];
         }
-        if (qn >= total) break;
+        auto store_rows = [&](auto loc, auto hic) {
+            constexpr int LO = decltype(loc)::value, HI = decltype(hic)::value;
+#pragma unroll
+            for (int i = LO; i < HI; ++i) dst.store(o[i], lane_out, (int)(i * n * sizeof(double)));
+        };
+        if (qn >= total) {
+            store_rows(std::integral_constant<int, 0>{}, std::integral_constant<int, ITER>{});
+            break;
+        }
         q = qn;
-        __syncthreads();   // the store phase has read the plane before the next one is staged
-        stage();
+        __syncthreads();   // every wave has read the plane before the next one is staged
+        static_assert(ITER % kStreamChunk == 0, "whole chunks");
+        static_for<0, ITER / kStreamChunk>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            using Lo = std::integral_constant<int, c * kStreamChunk>;
+            using Hi = std::integral_constant<int, (c + 1) * kStreamChunk>;
+            store_rows(Lo{}, Hi{});
+            stage_rows(Lo{}, Hi{});
+        });
     }
 }
 
