@@ -190,11 +190,15 @@ def test_group_uneven_eager_and_graph_replays(fdm, hip, srv):
     grp.close()
 
 
-def test_server_alone_vs_mix(fdm, hip, srv):
-    say("server alone vs mix")
+@pytest.mark.parametrize("nstreams", [4, 2, 1])
+def test_server_alone_vs_mix(fdm, hip, srv, nstreams):
+    """Results are bitwise the same alone and in the mix, with the default
+    four streams (the first, for large requests, on a CU mask), two, and one
+    stream shared by every size class (no mask)."""
+    say("server alone vs mix", nstreams)
     probs, keep = _uneven_problems(srv, hip, fdm, 12, seed=5)
     noise, nkeep = _uneven_problems(srv, hip, fdm, 12, seed=6)
-    S = srv.Server(hbm_budget=512 << 20, max_batch=8, nstreams=4)
+    S = srv.Server(hbm_budget=512 << 20, max_batch=8, nstreams=nstreams)
     alone = []
     for p, k in zip(probs, keep):
         t = S.submit(p)
