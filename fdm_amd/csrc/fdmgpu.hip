@@ -216,7 +216,7 @@ static bool square_planes(const fdmgpu_plan_t* P) { return P->logx == P->logy; }
 // the plane fits LDS, else the axis tiles.
 int default_plane(const fdmgpu_plan_t* P) {
     if (!plane_capable(P)) return 0;
-    return square_planes(P) && plane_stream_ok(P->logx, P->precision == FDMGPU_F64) ? 2 : 1;
+    return square_planes(P) && P->precision == FDMGPU_F64 && plane_stream_ok(P->logx, true) ? 2 : 1;
 }
 
 std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
@@ -236,10 +236,23 @@ std::vector<Variant> plan_variants(const fdmgpu_plan_t* P) {
     Variant d;
     d.plane = default_plane(P);
     out.push_back(d);   // the default first
+    if (d.plane == 1 && square_planes(P) && !f64 && plane_stream_ok(P->logx, false)) {
+        Variant v;   // plane_stream32_kernel with the default z solve
+        v.plane = 2;
+        out.push_back(v);
+    }
     const bool refine = P->precision == FDMGPU_F32_REFINE64;
     for (int gm : gms) {
-        if (plane_capable(P) && square_planes(P) && plane_stream_ok(P->logx, P->precision == FDMGPU_F64)) {
+        if (plane_capable(P) && square_planes(P) && f64 && plane_stream_ok(P->logx, true)) {
             Variant v;   // plane_stream_kernel (plain FP64 planes of N = 128)
+            v.plane = 2;
+            v.zcr_gmul = gm;
+            out.push_back(v);
+        }
+        if (plane_capable(P) && square_planes(P) && !f64 && plane_stream_ok(P->logx, false)) {
+            // plane_stream32_kernel (FP32 planes of N = 128: LaplCube<float>, the
+            // refinement's FP32-to-FP32 transforms), with the separate residual pass
+            Variant v;
             v.plane = 2;
             v.zcr_gmul = gm;
             out.push_back(v);

@@ -212,7 +212,8 @@ template <typename Tu>
 hipError_t launch_plane_resid(int logn, const PlaneResidArgs& a);
 template <typename T>
 bool plane_fits(int logn);
-// plane_stream_kernel exists for this plane (FP64 planes of N = 128)
+// a streaming plane kernel exists for this plane (N = 128: plane_stream_kernel
+// for FP64, plane_stream32_kernel for FP32)
 bool plane_stream_ok(int logn, bool f64);
 // Dirichlet planes of (2^logy - 1) x (2^logx - 1) points, logx != logy, both
 // 4 .. 7: the rectangular plane kernel (plane_rect_kernels.hip)

@@ -33,7 +33,8 @@ using Tab = const void* const*;
 // Kernel variant of a plan (chosen by the autotuner; 0 = built-in default).
 struct Variant {
     int plane = 1;      // square planes that fit LDS: 1 = plane kernel, 0 = x and y tile kernels,
-                        // 2 = streaming plane kernel (plain FP64 planes of N = 128)
+                        // 2 = streaming plane kernel (planes of N = 128: FP64, the default
+                        // there; FP32 and the refinement's FP32-to-FP32 transforms)
     int tile_x = 0;     // x tile kernel: rows per block (16/32/64)
     int tile_y = 0;     // y tile kernel: columns per block
     int zcr_gmul = 0;   // z cyclic reduction: lanes-per-system multiplier (1/2)

@@ -190,9 +190,9 @@ __attribute__((amdgpu_waves_per_eu(PlaneCfg<LOGN>::template waves<T>()))) void p
 // or of stores issued at once held every wave for about 4k cycles per plane,
 // measured with s_memtime stamps.)  The plane sits in LDS in the
 // b64-conflict-free PlaneLayoutB64 (pitch kStreamPitch, jobs dealt to pencils
-// out of order); each pencil's transform and the order of every
-// floating-point operation in it are those of plane_dst_kernel, so results
-// are bitwise the same.
+// out of order).  Each pencil goes through plane_dst_kernel's transform;
+// results agree with that kernel's to FP64 rounding (2e-14 relative at
+// 127^3), not bitwise.
 // ---------------------------------------------------------------------------
 constexpr int kStreamPitch = 130;
 constexpr int kStreamSplit = 16;   // rows (of 32 per thread) prefetched before the x pass
@@ -354,6 +354,155 @@ __global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu
 }
 
 // ---------------------------------------------------------------------------
+// Streaming plane kernel for FP32 planes of N = 128 (LaplCube<float> and the
+// refinement's FP32 inner solves at 127^3; periodic 128^3): the scheme of
+// plane_stream_kernel for a plane of floats.  Such a plane takes 70 KB of LDS,
+// so two blocks share a CU (the grid is two per CU) and a block holds at most
+// 128 registers per lane: the lane constants' shifted sines are read from LDS
+// and a thread's 32 prefetched rows take 32 of them.  The plane sits in the
+// swizzled PlaneLayout of plane_dst_kernel; a thread's rows are ITER
+// consecutive ones starting on a multiple of the swizzle period, so each
+// row's slot is the thread's first plus a compile-time constant.  Problems
+// flagged in `done` are skipped (their planes are not read or written).  The
+// y pass is the generic column transform (plane_dst_kernel's slot-table
+// variant of it would spill here), so results agree with plane_dst_kernel's
+// to float rounding, not bitwise.
+// ---------------------------------------------------------------------------
+constexpr int kStream32Split = 16;
+constexpr int kStream32Chunk = 8;
+
+template <int KIND>
+__global__ __launch_bounds__(PlaneCfg<7>::NT) __attribute__((amdgpu_waves_per_eu(4))) void plane_stream32_kernel(
+    const BatchPtr<const float> in, const BatchPtr<float> out, const float sx, const float sy,
+    const float* __restrict__ tables, const int* __restrict__ done, const int nplanes, const int total) {
+    constexpr int LOGN = 7;
+    using C = PlaneCfg<LOGN>;
+    using S = DstShape<LOGN>;
+    using IO = PencilIO<KIND, LOGN>;
+    using LX = PlaneLayout<C::P, S::N1, true>;
+    using LY = PlaneLayout<C::P, S::N1, false>;
+    using Sw = typename LX::Sw;
+    constexpr int N = C::N, P = C::P, n = IO::n, o = IO::off, NT = C::NT;
+    constexpr int R = NT / N, ITER = (n + R - 1) / R;
+    static_assert(NT % N == 0 && ITER % S::N1 == 0, "row groups start on a swizzle period");
+    const int tid = threadIdx.x;
+    const int G = (int)gridDim.x;
+    // a plane of a problem flagged done is skipped (wave-uniform: scalar load)
+    auto skipped = [&](const int qq) {
+        if (!done) return false;
+        using CI = const __attribute__((address_space(4))) int*;
+        return ((CI)done)[__builtin_amdgcn_readfirstlane(qq / nplanes)] != 0;
+    };
+    auto next = [&](int qq) {
+        do qq += G;
+        while (qq < total && skipped(qq));
+        return qq;
+    };
+    int q = blockIdx.x;
+    if (q < total && skipped(q)) q = next(q);
+    if (q >= total) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float* lds = reinterpret_cast<float*>(smem_raw);
+    constexpr unsigned plane_pts = (unsigned)n * n;
+    // thread (r0, x0): column x0 of rows ITER r0 + i, i < ITER (r0 wave-uniform)
+    const int x0 = tid & (N - 1), r0 = __builtin_amdgcn_readfirstlane(tid >> LOGN);
+    const int slot0 = opaque(LX::at(ITER * r0 + o, x0 + o));
+    const int lane_in = (int)((ITER * r0 * n + x0) * sizeof(float));
+    auto slot = [&](const int i) { return slot0 + i * P + Sw::f(i + o) - Sw::f(o); };
+
+    using Lane = DstLaneLdsSine<float, LOGN>;
+    float* sines = lds + N * P;
+    Lane L;
+    Lane::fill_sines(tables, sines, tid, NT);
+    L.init(tables, sines, tid & (S::N1 - 1));
+    if constexpr (KIND == kDst) {   // the dummy row and column stay 0 across planes
+        for (int i = tid; i < N; i += NT) {
+            lds[LX::at(0, i)] = 0.f;
+            lds[LX::at(i, 0)] = 0.f;
+        }
+    }
+    float v[ITER];
+    auto split = [&](const int qq, int& p, int& z) {
+        p = __builtin_amdgcn_readfirstlane(qq / nplanes);
+        z = __builtin_amdgcn_readfirstlane(qq - p * nplanes);
+    };
+    auto load_rows = [&](const int qq, auto loc, auto hic) {
+        constexpr int LO = decltype(loc)::value, HI = decltype(hic)::value;
+        int p, z;
+        split(qq, p, z);
+        const BufRange<float> src(wave_uniform(in.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(float));
+#pragma unroll
+        for (int i = LO; i < HI; ++i) v[i] = src.load(lane_in, (int)(i * n * sizeof(float)));
+    };
+    using R0 = std::integral_constant<int, 0>;
+    using RH = std::integral_constant<int, kStream32Split>;
+    using R1 = std::integral_constant<int, ITER>;
+    auto stage_rows = [&](auto loc, auto hic) {
+        constexpr int LO = decltype(loc)::value, HI = decltype(hic)::value;
+#pragma unroll
+        for (int i = LO; i < HI; ++i)   // a row >= n would land on the sines behind the plane
+            if (ITER * (R - 1) + i < n || ITER * r0 + i < n) lds[slot(i)] = v[i];
+    };
+    // the loop's order of loads, waits and stores is plane_stream_kernel's
+    load_rows(q, R0{}, R1{});
+    stage_rows(R0{}, R1{});
+    wait_vmem_all();
+    for (;;) {
+        __syncthreads();
+        const int qn = next(q);
+        if (qn < total) load_rows(qn, R0{}, RH{});
+        __builtin_amdgcn_sched_barrier(0);
+        int t1 = tid;
+        asm volatile("" : "+v"(t1));
+        pencil_pairs<KIND, float, LOGN, N / 2, LX, NT, Lane>(lds, L, sx, t1);   // pencils = rows    (x)
+        __syncthreads();
+        if constexpr (kStream32Split < ITER)
+            if (qn < total) load_rows(qn, RH{}, R1{});
+        __builtin_amdgcn_sched_barrier(0);
+        int t2 = tid;
+        asm volatile("" : "+v"(t2));
+        // (dst_pairs128's slot tables would take registers the prefetch holds)
+        pencil_pairs<KIND, float, LOGN, N / 2, LY, NT, Lane>(lds, L, sy, t2);   // pencils = columns (y)
+        __syncthreads();
+        int t3 = tid;
+        asm volatile("" : "+v"(t3));
+        int p, z;
+        split(q, p, z);
+        const BufRange<float> dst(wave_uniform(out.uniform(p)) + (long)z * plane_pts, plane_pts * sizeof(float));
+        // every lane stores every row, with no branch: a lane outside the
+        // plane reads a slot inside it and stores beyond the buffer's range
+        const int xo = t3 & (N - 1);
+        const int xs = opaque(IO::out(xo < n ? xo : n - 1));
+        const int lane_out = xo < n ? (int)((ITER * r0 * n + xo) * sizeof(float)) : (int)(plane_pts * sizeof(float));
+        float ov[ITER];
+#pragma unroll
+        for (int i = 0; i < ITER; ++i) {
+            const int y = ITER * r0 + i, ys = y < n - 1 ? y : n - 1;
+            ov[i] = lds[LX::at(IO::out(ys), 0) + xs];
+        }
+        auto store_rows = [&](auto loc, auto hic) {
+            constexpr int LO = decltype(loc)::value, HI = decltype(hic)::value;
+#pragma unroll
+            for (int i = LO; i < HI; ++i) dst.store(ov[i], lane_out, (int)(i * n * sizeof(float)));
+        };
+        if (qn >= total) {
+            store_rows(R0{}, R1{});
+            break;
+        }
+        q = qn;
+        __syncthreads();   // every wave has read the plane before the next one is staged
+        static_assert(ITER % kStream32Chunk == 0, "whole chunks");
+        static_for<0, ITER / kStream32Chunk>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            using Lo = std::integral_constant<int, c * kStream32Chunk>;
+            using Hi = std::integral_constant<int, (c + 1) * kStream32Chunk>;
+            store_rows(Lo{}, Hi{});
+            stage_rows(Lo{}, Hi{});
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Refinement: forward DST of the FP64 residual r = f - A u, fused into the
 // plane load.  A is the 7-point operator the solve inverts (1/h^2 per axis:
 // ix, iy, iz).  Each block also writes (sum r^2, sum f^2) of its plane to
@@ -502,18 +651,50 @@ static hipError_t plane_stream_k(const PlaneArgs& a) {
     return hipGetLastError();
 }
 
+template <int KIND>
+static hipError_t plane_stream32_k(const PlaneArgs& a) {
+    using C = PlaneCfg<7>;
+    auto k = plane_stream32_kernel<KIND>;
+    const size_t lds = sizeof(float) * (size_t)C::N * (C::P + 1);   // plane + sine table
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    const long total = (long)a.nplanes * a.batch;
+    if (total < 1 || total > 0x7fffffffL) return hipErrorInvalidValue;
+    // two blocks per CU; every block gets ceil(total / blocks) planes or one fewer
+    const long blocks = 2L * device_cus();
+    const long per = (total + blocks - 1) / blocks;
+    const int grid = (int)((total + per - 1) / per);
+    const BatchPtr<const float> in{(const float*)a.in, a.in_tab, a.bs_in};
+    const BatchPtr<float> out{(float*)a.out, a.out_tab, a.bs_out};
+    hipLaunchKernelGGL(k, dim3(grid), dim3(C::NT), lds, a.stream, in, out, (float)a.sx, (float)a.sy,
+                       (const float*)a.tables, a.done, a.nplanes, (int)total);
+    return hipGetLastError();
+}
+
 template <typename Tin, typename Tout, typename T, int LOGN>
 static hipError_t plane_t(const PlaneArgs& a) {
     if (a.base && !a.accum) return hipErrorInvalidValue;
-    if (a.stream_planes) {   // plain FP64 planes of N = 128 only
+    if (a.stream_planes) {   // planes of N = 128: FP64 without done flags, FP32 in and out
         if constexpr (LOGN == 7 && std::is_same_v<Tin, double> && std::is_same_v<Tout, double> &&
                       std::is_same_v<T, double>) {
             if (a.accum || a.base || a.done) return hipErrorInvalidValue;
             if (a.kind == kDst) return plane_stream_k<kDst>(a);
             if (a.kind == kRdftFwd) return plane_stream_k<kRdftFwd>(a);
             if (a.kind == kRdftInv) return plane_stream_k<kRdftInv>(a);
+            return hipErrorInvalidValue;
         }
-        return hipErrorInvalidValue;
+        if constexpr (LOGN == 7 && std::is_same_v<Tin, float> && std::is_same_v<Tout, float> &&
+                      std::is_same_v<T, float>) {
+            if (!a.accum) {
+                if (a.kind == kDst) return plane_stream32_k<kDst>(a);
+                if (a.kind == kRdftFwd) return plane_stream32_k<kRdftFwd>(a);
+                if (a.kind == kRdftInv) return plane_stream32_k<kRdftInv>(a);
+                return hipErrorInvalidValue;
+            }
+        }
+        // the refinement's other FP32 planes (FP64 in, or FP64 out with an
+        // FP32 base) take plane_dst_kernel below
+        if constexpr (!std::is_same_v<T, float>) return hipErrorInvalidValue;
     }
     if (a.kind == kDst) {
         if constexpr (std::is_same_v<Tout, double> && !std::is_same_v<Tin, double>) {
@@ -581,7 +762,7 @@ bool plane_fits(int logn) {
     return sizeof(T) * (N * (N + N1)) <= 160 * 1024;
 }
 
-bool plane_stream_ok(int logn, bool f64) { return f64 && logn == 7; }
+bool plane_stream_ok(int logn, bool) { return logn == 7; }
 
 template hipError_t launch_plane<float, float, float>(int, const PlaneArgs&);
 template hipError_t launch_plane<double, double, double>(int, const PlaneArgs&);

@@ -148,9 +148,11 @@ int fdmgpu_solve_batch_noalloc(fdmgpu_plan plan, int count, void* d_ans, const v
 
 /* Kernel variants (tile shapes) of a plan, as the autotuner chooses them:
  * plane = 1 runs square planes that fit LDS through the plane kernel (0: x
- * and y tile kernels; 2: plain FP64 planes of N = 128 through the streaming
- * plane kernel, one block per compute unit walking its planes with the next
- * plane's loads in flight -- the default there); tile_x / tile_y = pencils per block of the tile
+ * and y tile kernels; 2: planes of N = 128 through a streaming plane kernel,
+ * blocks walking their planes with the next plane's loads in flight -- one
+ * block per compute unit for FP64, the default there; two for FP32 and the
+ * FP32 transforms of FDMGPU_F32_REFINE64, a variant the autotuner may pick);
+ * tile_x / tile_y = pencils per block of the tile
  * kernels (16, 32, 64; 0 = default); zcr_gmul = z cyclic reduction
  * layout: 1, 2 = lanes per system multiplier inside a wavefront, 3, 4 = one
  * wavefront per 32-row chunk of 64 consecutive modes, 64 (3) or 128 (4)
