@@ -295,7 +295,9 @@ class _Plan:
         return out
 
     def variants(self):
-        """Kernel variants valid for this plan (tuples plane, tile_x, tile_y, zcr_gmul)."""
+        """Kernel variants valid for this plan, the default first: tuples (plane, tile_x,
+        tile_y, zcr_gmul, fuse), plane 0 = x and y tile kernels, 1 = plane kernel,
+        2 = streaming plane kernel (planes of N = 128), as fdmgpu_variant in fdmgpu.h."""
         L = _lib_handle()
         n = C.c_int()
         _check(L.fdmgpu_plan_variants(self._h, 0, None, C.byref(n)), "plan_variants")
