@@ -418,18 +418,20 @@ def test_f64_streaming_planes_vs_plane_kernel(fdm, periodic):
 @pytest.mark.parametrize("periodic", [False, True])
 def test_f32_streaming_planes_vs_plane_kernel(fdm, periodic):
     """The streaming FP32 plane kernel (variant plane 2 of FP32 plans with planes
-    of N = 128) against the one-plane-per-block kernel on a batch of 3 with 64
-    planes each (blocks walk planes of different problems): both within 2e-6 of
-    the FP64 oracle and 1e-6 of each other (the y pass differs in operation
-    order: float rounding); the same problems through a reversed pointer table
-    bitwise those of the batch."""
+    of N = 128) against the one-plane-per-block kernel on a batch of 5 with 255
+    or 256 planes each: with two blocks per CU every block walks three planes
+    (256 CUs), of different problems.  Both within 2e-6 of the FP64 oracle and
+    1e-6 of each other (the y pass differs in operation order: float
+    rounding); the same problems through a reversed pointer table bitwise
+    those of the batch."""
     from fdm_amd import hip
-    n, nz = (128, 64) if periodic else (127, 63)
-    say("f32 streaming planes", n, nz, "periodic" if periodic else "dirichlet")
+    n, nz = (128, 256) if periodic else (127, 255)
+    B = 5
+    say("f32 streaming planes", n, nz, B, "periodic" if periodic else "dirichlet")
     h = 2 * np.pi / n
     l = n * h if periodic else (n + 1) * h
     lz = nz * h if periodic else (nz + 1) * h
-    F = np.stack([O.field(70 + k, (nz, n, n)) for k in range(3)])
+    F = np.stack([O.field(70 + k, (nz, n, n)) for k in range(B)])
     if periodic:
         F -= F.mean(axis=(1, 2, 3), keepdims=True)
     F32 = F.astype(np.float32)
@@ -444,16 +446,17 @@ def test_f32_streaming_planes_vs_plane_kernel(fdm, periodic):
         s.solve_batch(d_u, d_f)
         hip.synchronize()
         out[v[0]] = d_u.download()
-    for i in range(3):
+    for i in range(B):
         assert rel2(out[2][i], out[1][i]) <= 1e-6, i
-    fs = [hip.DeviceArray.from_host(F32[i]) for i in range(3)]
-    us = [hip.DeviceArray(F32[i].shape, np.float32) for i in range(3)]
+    fs = [hip.DeviceArray.from_host(F32[i]) for i in range(B)]
+    us = [hip.DeviceArray(F32[i].shape, np.float32) for i in range(B)]
     s.variant = stream
-    tab_u = hip.DeviceArray.from_host(np.array([us[i].ptr.value for i in (2, 1, 0)], np.uint64))
-    tab_f = hip.DeviceArray.from_host(np.array([fs[i].ptr.value for i in (2, 1, 0)], np.uint64))
-    s.solve_table(tab_u, tab_f, 3)
+    rev = list(range(B))[::-1]
+    tab_u = hip.DeviceArray.from_host(np.array([us[i].ptr.value for i in rev], np.uint64))
+    tab_f = hip.DeviceArray.from_host(np.array([fs[i].ptr.value for i in rev], np.uint64))
+    s.solve_table(tab_u, tab_f, B)
     hip.synchronize()
-    for i in range(3):
+    for i in range(B):
         assert np.array_equal(us[i].download(), out[2][i]), i
     ref = O.lapl_cube(F32[1].astype(np.float64), h, h, h, l, l, lz, periodic=periodic)
     assert rel2(out[2][1], ref) <= 2e-6
@@ -463,18 +466,20 @@ def test_f32_streaming_planes_vs_plane_kernel(fdm, periodic):
 @pytest.mark.parametrize("periodic", [False, True])
 def test_refine_streaming_planes_with_done_flags(fdm, periodic):
     """FP32+refine through the streaming FP32 plane kernel and through the plane
-    kernel: a batch of 3 where one problem is certified after 2 inner solves
-    and the others take a third (whose transforms skip the certified problem's
-    planes).  Per variant, every answer and iteration count in the batch is
-    bitwise that of the problem solved alone, and within 1e-9 of the FP64
-    oracle."""
+    kernel: a batch of 5 with 255 or 256 planes each (every block walks three
+    planes, of different problems) where one problem is certified after 2
+    inner solves and the others take a third, whose transforms skip the
+    certified problem's planes wherever they fall in a block's sequence.  Per
+    variant, every answer and iteration count in the batch is bitwise that of
+    the problem solved alone, and within 1e-9 of the FP64 oracle."""
     from fdm_amd import hip
-    n, nz = (128, 32) if periodic else (127, 31)
-    say("refine streaming planes, done flags", n, nz, "periodic" if periodic else "dirichlet")
+    n, nz = (128, 256) if periodic else (127, 255)
+    B = 5
+    say("refine streaming planes, done flags", n, nz, B, "periodic" if periodic else "dirichlet")
     h = 2 * np.pi / n
     l = n * h if periodic else (n + 1) * h
     lz = nz * h if periodic else (nz + 1) * h
-    F = np.stack([O.field(50 + k, (nz, n, n)) for k in range(3)])
+    F = np.stack([O.field(50 + k, (nz, n, n)) for k in range(B)])
     if periodic:
         F -= F.mean(axis=(1, 2, 3), keepdims=True)
     s = fdm.LaplCube(h, h, h, l, l, lz, n, n, nz, precision="refine", periodic=periodic)
@@ -488,7 +493,7 @@ def test_refine_streaming_planes_with_done_flags(fdm, periodic):
         # the residual of each problem after 2 inner solves
         s.set_refine(1e-30, 2)
         r2 = []
-        for i in range(3):
+        for i in range(B):
             with pytest.raises(fdm.FdmGpuError):
                 s.solve(np.zeros_like(F[i]), F[i])
             r2.append(s.stats().rel_residual)
@@ -496,8 +501,9 @@ def test_refine_streaming_planes_with_done_flags(fdm, periodic):
         lo, mid = r2[order[0]], r2[order[1]]
         assert mid > lo * 1.01, r2
         s.set_refine(float(np.sqrt(lo * mid)), 4)   # problem order[0] stops after 2 solves, the others after 3
+        say("    certified after 2 solves:", int(order[0]), "r2", r2)
         alone = []
-        for i in range(3):
+        for i in range(B):
             ans = np.zeros_like(F[i])
             s.solve(ans, F[i])
             assert s.stats().iterations == (2 if i == order[0] else 3), (i, r2)
@@ -506,7 +512,7 @@ def test_refine_streaming_planes_with_done_flags(fdm, periodic):
         s.solve_batch(d_u, d_f)
         hip.synchronize()
         ans = d_u.download()
-        for i in range(3):
+        for i in range(B):
             assert np.array_equal(ans[i], alone[i]), (v, i)
         assert rel2(ans[2], ref) <= 1e-9
 
