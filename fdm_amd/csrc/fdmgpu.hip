@@ -295,10 +295,7 @@ bool variant_valid(const fdmgpu_plan_t* P, const Variant& v) {
 
 namespace {
 
-// Records HIP events around one launch when timing is enabled.  The events
-// only carry timestamps, so they are created without the system-scope fence
-// a default event performs at each record (HIP's flag for timing-only
-// events; measured no change in the timed solves either way).
+// Records HIP events around one launch when timing is enabled.
 struct Timed {
     fdmgpu_plan_t* P;
     size_t slot = 0;
@@ -309,11 +306,7 @@ struct Timed {
         slot = P->nrec++;
         while (P->evs.size() < 2 * P->nrec) {
             hipEvent_t e;
-            if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
-                on = false;
-                P->nrec--;
-                return;
-            }
+            if (hipEventCreate(&e) != hipSuccess) { on = false; P->nrec--; return; }
             P->evs.push_back(e);
         }
         if (P->rec_kernel.size() < P->nrec) P->rec_kernel.resize(P->nrec);

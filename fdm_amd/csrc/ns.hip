@@ -511,8 +511,8 @@ int fdmgpu_ns_step(fdmgpu_ns N, int count, void* stream) {
 int fdmgpu_ns_step_timed(fdmgpu_ns N, double* ms, void* stream) {
     if (!N || !ms) return FDMGPU_ERR_INVALID_ARG;
     NS_HIPCHK(hipSetDevice(N->device));
-    for (hipEvent_t& e : N->ev_phase)   // timestamps only: no system-scope fence at each record
-        if (!e) NS_HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    for (hipEvent_t& e : N->ev_phase)
+        if (!e) NS_HIPCHK(hipEventCreate(&e));
     hipStream_t user = (hipStream_t)stream;
     hipStream_t s = N->cap;
     NS_HIPCHK(hipEventRecord(N->ev_in, user));
